@@ -1,0 +1,438 @@
+"""ctypes/numpy front end of the FP64 CPU oracle (oracle/scarl_oracle.c).
+
+TEST INFRASTRUCTURE ONLY.  Imported by tests/, by ``__graft_entry__.smoke()`` and by the
+``cpu_baseline`` leg of bench.py -- as the checker, never as the thing measured or shipped.
+The product path (``scarl_amd``) never imports this module.
+
+Every 3D array is Fortran-ordered (x fastest), matching the reference's k_mtx layout
+(SRC_F/main_ScaRL.f90:405).  Functions with no independent pin are listed in the module
+``UNPINNED`` tuple; everything else is pinned by tests/test_oracle_*.py.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_PATH = os.path.join(HERE, "_ref", "libscarl_ref.so")
+
+CM_GAUSSIAN, CM_VON_KARMAN, CM_EXPONENTIAL = 1, 2, 3
+FOM_GAUSSIAN, FOM_LOGNORMAL = 1, 2
+
+# Behaviour the reference leaves undefined; the oracle's choice cannot be pinned.
+UNPINNED = (
+    "unwritten Sk_mtx octants are zero (SRC_F/fftw_ScaRL.f90:317,370)",
+    "Shinozuka consumer x extent sz1/2 (non-conformant assignment, fftw_ScaRL.f90:217)",
+)
+
+_lib = None
+_ref = None
+
+_D = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_DF = np.ctypeslib.ndpointer(dtype=np.float64, flags="F_CONTIGUOUS")
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", HERE, "all"], check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        i32p = C.POINTER(C.c_int32)
+        L.oracle_r8_uniform_01.restype = C.c_double
+        L.oracle_r8_uniform_01.argtypes = [i32p]
+        L.oracle_r8vec_uniform_01.argtypes = [C.c_int32, i32p, vp]
+        L.oracle_r8vec_normal_01.argtypes = [C.c_int32, i32p, vp]
+        L.oracle_r8vec_normal_01.restype = C.c_int
+        L.oracle_philox4x32_10.argtypes = [vp, vp, vp]
+        L.oracle_gauss_pair.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, vp]
+        L.oracle_uniform53.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32]
+        L.oracle_uniform53.restype = C.c_double
+        L.oracle_fft1d.argtypes = [C.c_int, vp, C.c_int]
+        L.oracle_dft1d_direct.argtypes = [C.c_int, vp, vp, C.c_int]
+        L.oracle_redft01_direct.argtypes = [C.c_int, vp, vp]
+        L.oracle_redft01_fast.argtypes = [C.c_int, vp, vp]
+        L.oracle_redft01_3d.argtypes = [vp, vp, vp, vp, C.c_int]
+        L.oracle_mode_r_amplitude.argtypes = [vp, vp, vp, vp, vp, C.c_int32, vp]
+        L.oracle_mode_r_amplitude.restype = C.c_int
+        L.oracle_gen_std_gauss_fft.argtypes = [vp, vp, vp, vp, C.c_int32, vp, C.c_int]
+        L.oracle_gen_std_gauss_fft.restype = C.c_int
+        L.oracle_gen_std_gauss_fft_mpi.argtypes = [vp, vp, vp, vp, C.c_int32, vp, C.c_int]
+        L.oracle_gen_std_gauss_fft_mpi.restype = C.c_int
+        L.oracle_gamma.argtypes = [C.c_double]
+        L.oracle_gamma.restype = C.c_double
+        L.oracle_kv.argtypes = [C.c_double, C.c_double]
+        L.oracle_kv.restype = C.c_double
+        L.oracle_spectrum.argtypes = [C.c_int, C.c_int, vp, vp, C.c_double]
+        L.oracle_spectrum.restype = C.c_double
+        L.oracle_target_corr.argtypes = [C.c_int, C.c_int, vp, vp, C.c_double]
+        L.oracle_target_corr.restype = C.c_double
+        L.oracle_noise_hermitian.argtypes = [vp, vp, C.c_uint64, C.c_uint32]
+        L.oracle_spectrum_sum.argtypes = [vp, vp, C.c_int, vp, C.c_double]
+        L.oracle_spectrum_sum.restype = C.c_double
+        L.oracle_amplitude_h.argtypes = [vp, vp, vp, C.c_int, vp, C.c_double, C.c_double, C.c_int]
+        L.oracle_c2r_3d.argtypes = [vp, vp, vp, C.c_int]
+        L.oracle_marginal.argtypes = [vp, C.c_long, C.c_int32, C.c_double, C.c_double]
+        L.oracle_marginal.restype = C.c_int
+        L.oracle_marginal_params.argtypes = [C.c_int32, C.c_double, C.c_double, vp, vp]
+        L.oracle_marginal_params.restype = C.c_int
+        L.oracle_gen_mode_h.argtypes = [vp, vp, vp, C.c_int, vp, C.c_double, C.c_int, vp, C.c_uint64,
+                                        C.c_uint32, C.c_int32, C.c_double, C.c_double, C.c_int]
+        L.oracle_gen_mode_h.restype = C.c_int
+        L.oracle_pou_axis_weight.argtypes = [C.c_int] * 6
+        L.oracle_pou_axis_weight.restype = C.c_double
+        L.oracle_apply_pou.argtypes = [vp, vp, vp, vp, vp]
+        L.oracle_moments.argtypes = [vp, C.c_long, C.c_int, vp, vp, vp, vp, C.c_int]
+        L.oracle_lag_sums.argtypes = [vp, vp, C.c_int, vp, vp]
+        L.oracle_build_sk_shape.argtypes = [C.c_int32, vp, vp, vp, vp]
+        L.oracle_build_sk_shape.restype = C.c_int
+        L.oracle_build_sk.argtypes = [C.c_int32, vp, vp, vp, C.c_int32, vp]
+        L.oracle_build_sk.restype = C.c_int
+        L.oracle_gen_std_gauss_shino_fft.argtypes = [vp, vp, vp, vp, C.c_int32, C.c_int32, C.c_int]
+        L.oracle_gen_std_gauss_shino_fft.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def ref_lib():
+    """The reference's own LCG and ikv compiled from source (oracle/_ref), or None."""
+    global _ref
+    if _ref is None and os.path.exists(REF_PATH):
+        R = C.CDLL(REF_PATH)
+        R.ref_ikv.argtypes = [C.c_double, C.c_double, C.c_int, C.c_void_p]
+        R.ref_r8vec_uniform_01.argtypes = [C.c_int, C.POINTER(C.c_int), C.c_void_p]
+        R.ref_r8vec_normal_01.argtypes = [C.c_int, C.POINTER(C.c_int), C.c_void_p]
+        _ref = R
+    return _ref
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _i3(v):
+    return np.ascontiguousarray(np.asarray(v, dtype=np.int32).reshape(3))
+
+
+def _d3(v):
+    return np.ascontiguousarray(np.asarray(v, dtype=np.float64).reshape(3))
+
+
+# --------------------------------------------------------------------------------------
+# LCG (SRC_F/normal_ScaRL.f90)
+# --------------------------------------------------------------------------------------
+def r8vec_uniform_01(n: int, seed: int):
+    s = C.c_int32(seed)
+    r = np.empty(n, np.float64)
+    lib().oracle_r8vec_uniform_01(n, C.byref(s), _p(r))
+    return r, s.value
+
+
+def r8vec_normal_01(n: int, seed: int):
+    s = C.c_int32(seed)
+    x = np.empty(max(n, 1), np.float64)
+    rc = lib().oracle_r8vec_normal_01(n, C.byref(s), _p(x))
+    if rc:
+        raise ValueError("R8VEC_NORMAL_01 - Fatal error: R8_UNIFORM_01 returned 0")
+    return x[:n], s.value
+
+
+# --------------------------------------------------------------------------------------
+# noise
+# --------------------------------------------------------------------------------------
+def philox4x32_10(ctr, key):
+    c = np.asarray(ctr, np.uint32).copy()
+    k = np.asarray(key, np.uint32).copy()
+    o = np.empty(4, np.uint32)
+    lib().oracle_philox4x32_10(_p(c), _p(k), _p(o))
+    return o
+
+
+def gauss_pair(idx, seed, stream=0):
+    g = np.empty(2, np.float64)
+    lib().oracle_gauss_pair(idx, seed, stream, _p(g))
+    return g
+
+
+def noise_hermitian(n, seed, stream=0):
+    """Complex Hermitian half-spectrum noise, shape (n1//2+1, n2, n3) Fortran order."""
+    n = [int(v) for v in n]
+    W = np.empty((n[0] // 2 + 1, n[1], n[2]), np.complex128, order="F")
+    lib().oracle_noise_hermitian(_p(W), _p(_i3(n)), seed, stream)
+    return W
+
+
+def phi_octant(m, seed, stream=0):
+    """Mode R phase array phi in [0,1) of shape m (Fortran), 53-bit Philox uniforms."""
+    m = [int(v) for v in m]
+    tot = m[0] * m[1] * m[2]
+    f = lib().oracle_uniform53
+    out = np.fromiter((f(i, seed, stream) for i in range(tot)), np.float64, count=tot)
+    return out.reshape(m, order="F")
+
+
+# --------------------------------------------------------------------------------------
+# transforms
+# --------------------------------------------------------------------------------------
+def fft1d(x, sign=+1):
+    a = np.ascontiguousarray(np.asarray(x, np.complex128)).copy()
+    lib().oracle_fft1d(a.size, _p(a), sign)
+    return a
+
+
+def dft1d_direct(x, sign=+1):
+    a = np.ascontiguousarray(np.asarray(x, np.complex128))
+    o = np.empty_like(a)
+    lib().oracle_dft1d_direct(a.size, _p(a), _p(o), sign)
+    return o
+
+
+def redft01_direct(x):
+    a = np.ascontiguousarray(np.asarray(x, np.float64))
+    y = np.empty_like(a)
+    lib().oracle_redft01_direct(a.size, _p(a), _p(y))
+    return y
+
+
+def redft01_fast(x):
+    a = np.ascontiguousarray(np.asarray(x, np.float64))
+    y = np.empty_like(a)
+    lib().oracle_redft01_fast(a.size, _p(a), _p(y))
+    return y
+
+
+def redft01_3d(a, m=None, nthreads=0):
+    a = np.asfortranarray(a, dtype=np.float64)
+    n = _i3(a.shape)
+    mm = _i3(a.shape if m is None else m)
+    out = np.empty_like(a, order="F")
+    lib().oracle_redft01_3d(_p(n), _p(mm), _p(a), _p(out), nthreads)
+    return out
+
+
+def c2r_3d(H, n, nthreads=0):
+    Hc = np.asfortranarray(H, dtype=np.complex128).copy(order="F")
+    out = np.empty(tuple(int(v) for v in n), np.float64, order="F")
+    lib().oracle_c2r_3d(_p(_i3(n)), _p(Hc), _p(out), nthreads)
+    return out
+
+
+# --------------------------------------------------------------------------------------
+# generators
+# --------------------------------------------------------------------------------------
+def gen_std_gauss_fft(Np, xRange, corrL, corrMod, phi, nthreads=0):
+    """gen_std_gauss_FFT (SRC_F/fftw_ScaRL.f90:261-418) on host-exported phi."""
+    Np = _i3(Np)
+    phi = np.asfortranarray(phi, dtype=np.float64)
+    assert phi.shape == tuple(int(v) // 2 for v in Np)
+    out = np.empty(tuple(Np), np.float64, order="F")
+    rc = lib().oracle_gen_std_gauss_fft(_p(out), _p(Np), _p(_d3(xRange)), _p(_d3(corrL)), corrMod, _p(phi), nthreads)
+    if rc:
+        raise ValueError("unsupported corrMod")
+    return out
+
+
+def gen_std_gauss_fft_mpi(L, xRange, corrL, corrMod, phi, nthreads=0):
+    """gen_Std_Gauss_FFT_MPI with one rank (SRC_F/fftw_ScaRL.f90:424-583)."""
+    L = _i3(L)
+    phi = np.asfortranarray(phi, dtype=np.float64)
+    assert phi.shape == (int(L[0]) // 2, int(L[1]) // 2, int(L[2]))
+    out = np.empty(tuple(L), np.float64, order="F")
+    rc = lib().oracle_gen_std_gauss_fft_mpi(_p(out), _p(L), _p(_d3(xRange)), _p(_d3(corrL)), corrMod, _p(phi),
+                                            nthreads)
+    if rc:
+        raise ValueError("unsupported corrMod")
+    return out
+
+
+def mode_r_amplitude(Np, m, xRange, corrL, corrMod, phi):
+    Np = _i3(Np)
+    phi = np.asfortranarray(phi, dtype=np.float64)
+    out = np.empty(tuple(Np), np.float64, order="F")
+    rc = lib().oracle_mode_r_amplitude(_p(out), _p(Np), _p(_i3(m)), _p(_d3(xRange)), _p(_d3(corrL)), corrMod,
+                                       _p(phi))
+    if rc:
+        raise ValueError("unsupported corrMod")
+    return out
+
+
+def gamma(x):
+    return lib().oracle_gamma(float(x))
+
+
+def kv(nu, x):
+    return lib().oracle_kv(float(nu), float(x))
+
+
+def spectrum(model, dim, kappa, L, nu=0.5):
+    return lib().oracle_spectrum(model, dim, _p(_d3(kappa)), _p(_d3(L)), float(nu))
+
+
+def target_corr(model, dim, r, L, nu=0.5):
+    return lib().oracle_target_corr(model, dim, _p(_d3(r)), _p(_d3(L)), float(nu))
+
+
+def spectrum_sum(n, dx, model, L, nu=0.5):
+    return lib().oracle_spectrum_sum(_p(_i3(n)), _p(_d3(dx)), model, _p(_d3(L)), float(nu))
+
+
+def amplitude_h(n, dx, model, L, nu=0.5, sigma=1.0, normalize=True):
+    n = [int(v) for v in n]
+    amp = np.empty((n[0] // 2 + 1, n[1], n[2]), np.float64, order="F")
+    lib().oracle_amplitude_h(_p(amp), _p(_i3(n)), _p(_d3(dx)), model, _p(_d3(L)), float(nu), float(sigma),
+                             int(normalize))
+    return amp
+
+
+def gen_mode_h(n, dx, model, L, nu=0.5, normalize=True, W=None, seed=0, stream=0, margi=0, avg=0.0, std=1.0,
+               nthreads=0):
+    """Hermitian C2R generator (unnormalised backward transform of amp*W) + marginal."""
+    n = [int(v) for v in n]
+    out = np.empty(n, np.float64, order="F")
+    if W is not None:
+        W = np.asfortranarray(W, dtype=np.complex128)
+        assert W.shape == (n[0] // 2 + 1, n[1], n[2])
+        wp = _p(W)
+    else:
+        wp = None
+    rc = lib().oracle_gen_mode_h(_p(out), _p(_i3(n)), _p(_d3(dx)), model, _p(_d3(L)), float(nu), int(normalize), wp,
+                                 seed, stream, margi, float(avg), float(std), nthreads)
+    if rc:
+        raise ValueError(f"oracle_gen_mode_h failed rc={rc}")
+    return out
+
+
+def marginal(x, margi, avg, std):
+    a = np.asfortranarray(x, dtype=np.float64).copy(order="F")
+    rc = lib().oracle_marginal(_p(a), a.size, margi, float(avg), float(std))
+    if rc:
+        raise ValueError(f"marginal rc={rc}")
+    return a
+
+
+def marginal_params(margi, avg, std):
+    mu = C.c_double()
+    sd = C.c_double()
+    rc = lib().oracle_marginal_params(margi, float(avg), float(std), C.byref(mu), C.byref(sd))
+    if rc:
+        raise ValueError(f"marginal rc={rc}")
+    return mu.value, sd.value
+
+
+def pou_axis_weight(i, np_, nov_lo, nov_hi, has_lo, has_hi):
+    return lib().oracle_pou_axis_weight(i, np_, nov_lo, nov_hi, int(has_lo), int(has_hi))
+
+
+def apply_pou(f, Nov, topo_pos, topo_shape):
+    a = np.asfortranarray(f, dtype=np.float64).copy(order="F")
+    lib().oracle_apply_pou(_p(a), _p(_i3(a.shape)), _p(_i3(Nov)), _p(_i3(topo_pos)), _p(_i3(topo_shape)))
+    return a
+
+
+def moments(rf, naive=False):
+    """rf: (n, nmc) -> (evnt_avg, evnt_std, g_avg, g_std)."""
+    rf = np.asfortranarray(rf, dtype=np.float64)
+    n, nmc = rf.shape
+    ea = np.empty(nmc)
+    es = np.empty(nmc)
+    ga = C.c_double()
+    gs = C.c_double()
+    lib().oracle_moments(_p(rf), n, nmc, _p(ea), _p(es), C.byref(ga), C.byref(gs), int(naive))
+    return ea, es, ga.value, gs.value
+
+
+def lag_sums(x, nlag):
+    x = np.asfortranarray(x, dtype=np.float64)
+    s = np.empty(3 * nlag)
+    c = np.empty(3 * nlag)
+    lib().oracle_lag_sums(_p(x), _p(_i3(x.shape)), nlag, _p(s), _p(c))
+    return s.reshape(3, nlag), c.reshape(3, nlag)
+
+
+def build_sk_shape(corrMod, xRange, Np, sz_c):
+    out = np.empty(3, np.int32)
+    rc = lib().oracle_build_sk_shape(corrMod, _p(_d3(xRange)), _p(_i3(Np)), _p(_i3(sz_c)), _p(out))
+    if rc:
+        raise ValueError("unsupported corrMod")
+    return out
+
+
+def build_sk(corrMod, xRange, Np, sz_c, seed):
+    sz = build_sk_shape(corrMod, xRange, Np, sz_c)
+    sk = np.empty((int(sz[0]), 2 * int(sz[1]), 2 * int(sz[2])), np.complex128, order="F")
+    lib().oracle_build_sk(corrMod, _p(_d3(xRange)), _p(_i3(Np)), _p(_i3(sz_c)), seed, _p(sk))
+    return sk
+
+
+def gen_std_gauss_shino_fft(Np, xRange, corrL, corrMod, seed, nthreads=0):
+    Np = _i3(Np)
+    out = np.empty(tuple(Np), np.float64, order="F")
+    rc = lib().oracle_gen_std_gauss_shino_fft(_p(out), _p(Np), _p(_d3(xRange)), _p(_d3(corrL)), corrMod, seed,
+                                              nthreads)
+    if rc:
+        raise ValueError("unsupported corrMod")
+    return out
+
+
+# --------------------------------------------------------------------------------------
+# numpy restatements (small, used for the blend / Stat estimator)
+# --------------------------------------------------------------------------------------
+def pou_weights_1d(ext, nov_lo, nov_hi, has_lo, has_hi):
+    return np.array([pou_axis_weight(i, ext, nov_lo, nov_hi, has_lo, has_hi) for i in range(ext)])
+
+
+def blend_global(fields, starts, gshape, weights_1d):
+    """Serial gather-sum of sqrt(w)-weighted subdomain fields into the global array
+    (the invariant of add_overlap, SRC_F/main_ScaRL.f90:984-1210)."""
+    G = np.zeros(gshape, np.float64, order="F")
+    for f, s, (wx, wy, wz) in zip(fields, starts, weights_1d):
+        w = np.sqrt(wx[:, None, None] * wy[None, :, None] * wz[None, None, :])
+        ex, ey, ez = f.shape
+        G[s[0]:s[0] + ex, s[1]:s[1] + ey, s[2]:s[2] + ez] += w * f
+    return G
+
+
+def rebuild_sk_corrl(field, xstep, avg, std):
+    """Stat corrL estimator restated with numpy FFTs (SRC_F/Stat_ScaRL/statistics_RF.f90:258-486),
+    including the 'Tukey' window exactly as written (:326-351) for 3D fields."""
+    x = np.asfortranarray(field, dtype=np.float64)
+    nd = x.ndim
+    pi = 3.1415926535898
+    Sk = []
+    for d in range(nd):
+        n = x.shape[d]
+        v = (x - avg) / std
+        if nd == 3:
+            t = np.ones(n)
+            for i in range(n):
+                ratio = i / (n - 1)
+                if ratio < 0.1:
+                    t[i] = 0.5 * (1 + np.cos(2 * pi / 0.1 * (ratio - 0.1 / 2)))
+                elif ratio > 0.1:
+                    t[i] = 0.5 * (1 + np.cos(2 * pi / 0.1 * (ratio - 1 + 0.1 / 2)))
+            shp = [1] * nd
+            shp[d] = n
+            v = v * t.reshape(shp)
+        F = np.fft.fft(v, axis=d) / n
+        P = (F * np.conj(F)).real
+        other = tuple(a for a in range(nd) if a != d)
+        m = P.mean(axis=other)
+        back = np.fft.ifft(m) * n  # FFTW_BACKWARD is unnormalised
+        Sk.append(back.real)
+    return Sk
+
+
+def corrl_from_sk(sk_half, xstep):
+    """rebuild_corrL (statistics_RF.f90:468-486): 2*(sum - (first+last)/2)*xStep."""
+    s = np.asarray(sk_half)
+    return 2.0 * (s.sum() - (s[0] + s[-1]) / 2.0) * xstep
