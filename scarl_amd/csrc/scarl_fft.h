@@ -1,0 +1,182 @@
+// scarl_fft.h -- one tile of pencils through a planned 1D backward DFT (device code).
+//
+// A tile is T pencils of one field handled by one 256-thread workgroup.  Stages run on
+// register-resident butterflies; LDS holds the tile between stages (in place: every stage
+// reads all its inputs, barriers, then writes).  Where the first stage can read its inputs
+// straight from HBM (plain loads) and the last stage can write straight to HBM (plain stores),
+// each pass costs exactly one HBM read and one HBM write.  Passes whose loader or epilogue is
+// heavy (white-noise generation, blend) stage through LDS instead (SIN / SOUT), so that code is
+// emitted once instead of once per unrolled element of every radix.
+#pragma once
+#include "scarl_device.h"
+#include "scarl_internal.h"
+
+namespace scarl {
+
+template <typename T>
+struct Ecap;
+template <>
+struct Ecap<float> {
+    static constexpr int v = kEcap32;
+};
+template <>
+struct Ecap<double> {
+    static constexpr int v = kEcap64;
+};
+
+// reference PI literal (SRC_F/constants_ScaRL.f90:5)
+constexpr double kPiRef = 3.1415926535898;
+
+//   gld(p, n) -> cplx<T> : element n of tile pencil p (pass input)
+//   gst(p, n, v)         : element n of tile pencil p (pass output)
+// SIN: load the whole tile into LDS first (one copy of gld).  SOUT: finish the transform in
+// LDS, then run gst from one loop.
+template <typename T, int LAYOUT, bool SIN, bool SOUT, class GLoad, class GStore>
+__device__ __forceinline__ void pencil_fft(const PlanDev& pl, const cplx<T>* __restrict__ pool, cplx<T>* lds,
+                                           int npen, GLoad&& gld, GStore&& gst) {
+    constexpr int EC = Ecap<T>::v;
+    const int tid = threadIdx.x;
+    const int Tt = pl.T, tpp = pl.tpp, N = pl.N, ldp = pl.ldp;
+    int p, b;
+    if constexpr (LAYOUT == LAYOUT_PF) {
+        p = tid % Tt;
+        b = tid / Tt;
+    } else {
+        b = tid % tpp;
+        p = tid / tpp;
+    }
+    const bool active = (p < npen) && (p < Tt) && (b < tpp);
+    auto addr = [&](int n) -> int {
+        if constexpr (LAYOUT == LAYOUT_PF) return n * Tt + p;
+        else return p * ldp + n + (n >> 4);
+    };
+    auto ld_g = [&](int n) -> cplx<T> { return gld(p, n); };
+    auto st_g = [&](int n, cplx<T> v) { gst(p, n, v); };
+    auto ld_l = [&](int n) -> cplx<T> { return lds[addr(n)]; };
+    auto st_l = [&](int n, cplx<T> v) { lds[addr(n)] = v; };
+
+    if (pl.nst == 0) {
+        // direct O(N^2) DFT (lengths with a prime factor above 13, and N = 1)
+#pragma unroll 1
+        for (int n = b; n < N; n += tpp)
+            if (active) st_l(n, ld_g(n));
+        __syncthreads();
+        const cplx<T>* tw = pool + pl.direct_off;
+#pragma unroll 1
+        for (int k = b; k < N; k += tpp) {
+            if (!active) break;
+            cplx<T> acc = {T(0), T(0)};
+            int t = 0;
+#pragma unroll 1
+            for (int j = 0; j < N; ++j) {
+                acc = acc + cmul(ld_l(j), tw[t]);
+                t += k;
+                if (t >= N) t -= N;
+            }
+            st_g(k, acc);
+        }
+        return;
+    }
+    if constexpr (SIN) {
+#pragma unroll 1
+        for (int n = b; n < N; n += tpp)
+            if (active) st_l(n, ld_g(n));
+        __syncthreads();
+    }
+    const int S = pl.nst;
+    // stage 0
+    if constexpr (SIN) {
+        if (S == 1 && !SOUT) stage_dispatch<EC, T>(pl.R[0], N, 1, pool, b, tpp, active, true, ld_l, st_g);
+        else stage_dispatch<EC, T>(pl.R[0], N, 1, pool, b, tpp, active, true, ld_l, st_l);
+    } else {
+        if (S == 1 && !SOUT) stage_dispatch<EC, T>(pl.R[0], N, 1, pool, b, tpp, active, false, ld_g, st_g);
+        else stage_dispatch<EC, T>(pl.R[0], N, 1, pool, b, tpp, active, false, ld_g, st_l);
+    }
+    if (S > 1) {
+        __syncthreads();
+        for (int s = 1; s < S - 1; ++s) {
+            stage_dispatch<EC, T>(pl.R[s], N, pl.Ns[s], pool + pl.tw_off[s], b, tpp, active, true, ld_l, st_l);
+            __syncthreads();
+        }
+        const int s = S - 1;
+        if constexpr (SOUT)
+            stage_dispatch<EC, T>(pl.R[s], N, pl.Ns[s], pool + pl.tw_off[s], b, tpp, active, true, ld_l, st_l);
+        else
+            stage_dispatch<EC, T>(pl.R[s], N, pl.Ns[s], pool + pl.tw_off[s], b, tpp, active, false, ld_l, st_g);
+    }
+    if constexpr (SOUT) {
+        __syncthreads();
+#pragma unroll 1
+        for (int n = b; n < N; n += tpp)
+            if (active) st_g(n, ld_l(n));
+    }
+}
+
+// Neumaier two-sum accumulation
+__device__ __forceinline__ void nadd(double& s, double& c, double v) {
+    const double t = s + v;
+    if (fabs(s) >= fabs(v)) c += (s - t) + v;
+    else c += (v - t) + s;
+    s = t;
+}
+
+// Block reduction of compensated pairs; red: 4 * kBlock doubles of LDS.
+__device__ __forceinline__ void block_reduce_mom(double s, double c, double s2, double c2, double* red,
+                                                 MomPartial* out) {
+    const int tid = threadIdx.x;
+    red[tid] = s;
+    red[tid + kBlock] = c;
+    red[tid + 2 * kBlock] = s2;
+    red[tid + 3 * kBlock] = c2;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (tid < w) {
+            double a = red[tid], ca = red[tid + kBlock];
+            nadd(a, ca, red[tid + w]);
+            ca += red[tid + w + kBlock];
+            red[tid] = a;
+            red[tid + kBlock] = ca;
+            double a2 = red[tid + 2 * kBlock], ca2 = red[tid + 3 * kBlock];
+            nadd(a2, ca2, red[tid + w + 2 * kBlock]);
+            ca2 += red[tid + w + 3 * kBlock];
+            red[tid + 2 * kBlock] = a2;
+            red[tid + 3 * kBlock] = ca2;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) *out = {red[0], red[kBlock], red[2 * kBlock], red[3 * kBlock]};
+}
+
+__device__ __forceinline__ int signed_freq(int m, int n) { return (m > n / 2) ? m - n : m; }
+
+template <typename T>
+__device__ __forceinline__ T pou_sqrt_weight(int i, int n, int nlo, int nhi) {
+    T w = T(1);
+    if (nlo >= 2 && i < nlo) {
+        const T t = T(nlo - 1 - i) / T(nlo - 1);
+        if constexpr (sizeof(T) == 4) w *= 0.5f * (1.0f + cospif(t));
+        else w *= 0.5 * (1.0 + cos(kPiRef * t));
+    }
+    if (nhi >= 2 && i >= n - nhi) {
+        const T t = T(i - (n - nhi)) / T(nhi - 1);
+        if constexpr (sizeof(T) == 4) w *= 0.5f * (1.0f + cospif(t));
+        else w *= 0.5 * (1.0 + cos(kPiRef * t));
+    }
+    return w;
+}
+
+__device__ __forceinline__ bool in_overlap(int i, int n, int nlo, int nhi) {
+    return (nlo > 0 && i < nlo) || (nhi > 0 && i >= n - nhi);
+}
+
+template <typename T>
+__device__ __forceinline__ T apply_margi(T v, int margi, T mu, T sd) {
+    if (margi == FOM_GAUSSIAN) return v * sd + mu;
+    if (margi == FOM_LOGNORMAL) {
+        if constexpr (sizeof(T) == 4) return expf(v * sd + mu);
+        else return exp(v * sd + mu);
+    }
+    return v;
+}
+
+}  // namespace scarl

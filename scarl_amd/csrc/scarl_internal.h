@@ -1,0 +1,91 @@
+// scarl_internal.h -- structures shared by the host runtime and the gfx950 kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace scarl {
+
+constexpr int kBlock = 256;      // threads per workgroup for every FFT pass
+constexpr int kMaxStages = 8;
+constexpr int kEcap32 = 32;      // complex elements a thread holds per stage (FP32)
+constexpr int kEcap64 = 16;      // (FP64)
+
+enum Layout { LAYOUT_PF = 0, LAYOUT_BF = 1 };  // pencil-fastest lanes / butterfly-fastest lanes
+
+// One 1D transform length, planned for one precision and one lane layout.
+struct PlanDev {
+    int N;                 // complex transform length
+    int nst;               // Stockham stages (0 = direct O(N^2) DFT fallback)
+    int R[kMaxStages];     // radices
+    int Ns[kMaxStages];    // product of the radices before each stage
+    int tw_off[kMaxStages];// per-stage twiddle table offset (complex elements) in the pool
+    int direct_off;        // direct-DFT table (N entries) offset, or -1
+    int pre_off;           // pre-twiddle table offset (C2R: exp(2 pi i k/(2N)), k<N; DCT: exp(i pi j/(2N)), j<N)
+    int tpp;               // threads per pencil
+    int T;                 // pencils per workgroup
+    int ldp;               // LDS pitch in complex elements (LAYOUT_BF)
+    int lds_elems;         // complex elements of LDS the tile needs
+};
+
+enum SourceMode { SRC_WORKSPACE = 0, SRC_DEVICE_NOISE = 1, SRC_HOST_NOISE = 2 };
+enum ModelCode { CM_GAUSSIAN = 1, CM_VON_KARMAN = 2, CM_EXPONENTIAL = 3 };
+enum MarginCode { FOM_NONE = 0, FOM_GAUSSIAN = 1, FOM_LOGNORMAL = 2 };
+
+// Mode H field (Hermitian half spectrum -> C2R) descriptor, device resident.
+struct FieldH {
+    int n[3];              // output extents x, y, z
+    int h1;                // n1/2 + 1
+    int plan_z, plan_y, plan_x;   // plan indices (z/y: LAYOUT_PF complex length n3/n2; x: LAYOUT_BF length n1/2)
+    int first_axis;        // 2 = z pass generates, 1 = y pass generates (2D)
+    int src_mode;          // SourceMode of the generating pass
+    int model;             // ModelCode
+    double alpha[3];       // kappa_d L_d per unit signed frequency = 2 pi L_d / (n_d dx_d)
+    double expo;           // Matern: -(nu + dim/2)/2 ; Gaussian: unused
+    double amp0;           // amplitude prefactor (FP64, includes the variance normalisation)
+    uint64_t seed;
+    uint32_t stream;
+    int pad0;
+    const void* W;         // host-exported noise (device copy), complex, half-spectrum layout
+    void* ws;              // complex half-spectrum workspace
+    void* out;             // output base (real)
+    int64_t out_ld1, out_ld2;  // output strides in elements along y and z
+    // blend (partition of unity over overlapping subdomains, fused into the x pass)
+    int color;             // bit d = parity of the subdomain position along axis d
+    int blend;             // 0: plain store; 1: PoU-weighted first/middle/last accumulation
+    int nov_lo[3], nov_hi[3];  // overlap lengths on sides that have a neighbour (0 = none)
+    // marginal + moments
+    int margi;
+    int mom_slot;          // -1 = no moments
+    double mu, sd;         // marginal parameters of the underlying Gaussian
+};
+
+// Mode R (REDFT01 reference parity, FP64) field descriptor.
+struct FieldR {
+    int n[3];              // output extents
+    int m[3];              // support of the octant amplitude (m_d <= n_d)
+    int plan[3];           // per-axis plan (x: LAYOUT_BF, y/z: LAYOUT_PF)
+    double dk[3];          // reference k spacing delta_k_d / corrL_d
+    double corrL[3];
+    int src_mode;          // SRC_DEVICE_NOISE (phi from Philox) or SRC_HOST_NOISE (phi array)
+    int margi;
+    double mu, sd;
+    uint64_t seed;
+    uint32_t stream;
+    int pad0;
+    const double* phi;     // (m1,m2,m3) phases
+    double* tmp1;          // (n1, m2, m3)
+    double* tmp2;          // (n1, n2, m3)
+    double* out;           // (n1, n2, n3)
+};
+
+struct Tile {
+    int field;
+    int pencil0;
+};
+
+// Compensated (Neumaier) partial sums of x and x^2 for one tile.
+struct MomPartial {
+    double s, c, s2, c2;
+};
+
+}  // namespace scarl

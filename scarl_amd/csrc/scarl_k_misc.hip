@@ -1,0 +1,221 @@
+// scarl_k_misc.hip -- reductions, element-wise operations, LCG jump-ahead, K_nu (gfx950).
+#include "scarl_fft.h"
+
+namespace scarl {
+
+// Per-field reduction of tile partials -> field_mom[f]; then per slot -> slot_out[slot]
+// slot_out[slot] = {sum, sum2, count, mean, std} with the reference's std formula
+// sqrt(sum2/n - mean^2) (SRC_F/Stat_ScaRL/statistics_RF.f90:80-82).
+__global__ __launch_bounds__(kBlock) void k_mom_field(const FieldH* __restrict__ fields,
+                                                       const int2* __restrict__ field_tiles,
+                                                       const MomPartial* __restrict__ partials,
+                                                       MomPartial* __restrict__ field_mom) {
+    __shared__ double red[4 * kBlock];
+    const int fi = blockIdx.x;
+    const FieldH& f = fields[fi];
+    double s = 0, c = 0, s2 = 0, c2 = 0;
+    if (f.mom_slot >= 0) {
+        const int2 r = field_tiles[fi];
+        for (int t = r.x + threadIdx.x; t < r.y; t += kBlock) {
+            const MomPartial m = partials[t];
+            nadd(s, c, m.s);
+            c += m.c;
+            nadd(s2, c2, m.s2);
+            c2 += m.c2;
+        }
+    }
+    block_reduce_mom(s, c, s2, c2, red, field_mom + fi);
+}
+
+__global__ __launch_bounds__(kBlock) void k_mom_slot(const FieldH* __restrict__ fields, int nfields,
+                                                      const MomPartial* __restrict__ field_mom,
+                                                      const double* __restrict__ slot_count, int nslots,
+                                                      double* __restrict__ slot_out) {
+    __shared__ double red[4 * kBlock];
+    const int sl = blockIdx.x;
+    double s = 0, c = 0, s2 = 0, c2 = 0;
+    for (int fi = threadIdx.x; fi < nfields; fi += kBlock) {
+        if (fields[fi].mom_slot != sl) continue;
+        const MomPartial m = field_mom[fi];
+        nadd(s, c, m.s);
+        c += m.c;
+        nadd(s2, c2, m.s2);
+        c2 += m.c2;
+    }
+    __shared__ MomPartial res;
+    block_reduce_mom(s, c, s2, c2, red, &res);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double n = slot_count[sl];
+        const double S = res.s + res.c, S2 = res.s2 + res.c2;
+        const double mean = S / n;
+        slot_out[5 * sl + 0] = S;
+        slot_out[5 * sl + 1] = S2;
+        slot_out[5 * sl + 2] = n;
+        slot_out[5 * sl + 3] = mean;
+        slot_out[5 * sl + 4] = sqrt(S2 / n - mean * mean);
+    }
+}
+
+// FP64 sum over the FULL spectrum of g(kappa) (Gaussian: exp(-q/4); Matern: (1+q)^(2 expo)),
+// half spectrum with multiplicity 2 off the self-conjugate planes.  One thread per (kx, ky)
+// column, loop over kz; Neumaier partials per block -> out[blockIdx.x].
+__global__ __launch_bounds__(kBlock) void k_spectrum_sum(FieldH f, MomPartial* __restrict__ out) {
+    __shared__ double red[4 * kBlock];
+    const int64_t col = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int n1 = f.n[0], n2 = f.n[1], n3 = f.n[2], h1 = f.h1;
+    double s = 0, c = 0;
+    if (col < (int64_t)h1 * n2) {
+        const int kx = (int)(col % h1), ky = (int)(col / h1);
+        const double mult = (kx == 0 || ((n1 & 1) == 0 && kx == n1 / 2)) ? 1.0 : 2.0;
+        const double a0 = f.alpha[0] * kx, a1 = f.alpha[1] * signed_freq(ky, n2);
+        for (int kz = 0; kz < n3; ++kz) {
+            const double a2 = f.alpha[2] * signed_freq(kz, n3);
+            const double q = a0 * a0 + a1 * a1 + a2 * a2;
+            const double g = (f.model == CM_GAUSSIAN) ? exp(-0.25 * q) : pow(1.0 + q, 2.0 * f.expo);
+            nadd(s, c, mult * g);
+        }
+    }
+    block_reduce_mom(s, c, 0.0, 0.0, red, out + blockIdx.x);
+}
+
+// Target correlation C(r)/sigma^2 (FP64, K_nu for von Karman): r[i] are anisotropic lags
+// already divided by L, as rho values.
+__global__ void k_target_corr(int model, double nu, const double* __restrict__ rho, int n, double* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double r = rho[i];
+    double v;
+    if (model == CM_GAUSSIAN) v = exp(-r * r);
+    else if (model == CM_EXPONENTIAL) v = exp(-r);
+    else v = (r == 0.0) ? 1.0 : pow(2.0, 1.0 - nu) / gamma_fp64(nu) * pow(r, nu) * kv_fp64(nu, r);
+    out[i] = v;
+}
+
+__global__ void k_kv(double nu, const double* __restrict__ x, int n, double* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = kv_fp64(nu, x[i]);
+}
+
+// ------------------------------------------------------------------------------------------
+// Standalone element-wise operations (FP64, device arrays)
+// ------------------------------------------------------------------------------------------
+// multiVariateTransformation (SRC_F/main_ScaRL.f90:1355-1395)
+__global__ void k_marginal(double* __restrict__ x, int64_t n, int margi, double mu, double sd) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double v = x[i] * sd + mu;
+        x[i] = (margi == FOM_LOGNORMAL) ? exp(v) : v;
+    }
+}
+
+// apply_UnityPartition_mtx (SRC_F/main_ScaRL.f90:1216-1301): f *= sqrt(prod_d w_d)
+__global__ void k_pou(double* __restrict__ x, int n1, int n2, int n3, int3 lo, int3 hi) {
+    const int64_t tot = (int64_t)n1 * n2 * n3;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < tot; t += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(t % n1);
+        const int64_t r = t / n1;
+        const int j = (int)(r % n2), k = (int)(r / n2);
+        const double w = pou_sqrt_weight<double>(i, n1, lo.x, hi.x) * pou_sqrt_weight<double>(j, n2, lo.y, hi.y) *
+                         pou_sqrt_weight<double>(k, n3, lo.z, hi.z);
+        x[t] *= sqrt(w);
+    }
+}
+
+// Compensated per-event sums of x and x^2: grid (blocks, nmc); partial per block
+__global__ __launch_bounds__(kBlock) void k_sums(const double* __restrict__ rf, int64_t n,
+                                                  MomPartial* __restrict__ partial) {
+    __shared__ double red[4 * kBlock];
+    const int e = blockIdx.y;
+    const double* x = rf + (int64_t)e * n;
+    double s = 0, c = 0, s2 = 0, c2 = 0;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+        const double v = x[i];
+        nadd(s, c, v);
+        nadd(s2, c2, v * v);
+    }
+    block_reduce_mom(s, c, s2, c2, red, partial + (int64_t)e * gridDim.x + blockIdx.x);
+}
+
+// LCG jump-ahead: seed_i = seed0 * 16807^(i+1) mod (2^31 - 1), r_i = seed_i * 4.656612875e-10
+// (SRC_F/normal_ScaRL.f90:1497-1512); powers by binary exponentiation on 64-bit products.
+__device__ __forceinline__ uint64_t mulmod_m31(uint64_t a, uint64_t b) {
+    const uint64_t m = 2147483647ull;
+    uint64_t p = a * b;                   // < 2^62
+    p = (p & m) + (p >> 31);
+    p = (p & m) + (p >> 31);
+    return p >= m ? p - m : p;
+}
+__device__ __forceinline__ uint64_t powmod_m31(uint64_t base, uint64_t e) {
+    uint64_t r = 1;
+    while (e) {
+        if (e & 1) r = mulmod_m31(r, base);
+        base = mulmod_m31(base, base);
+        e >>= 1;
+    }
+    return r;
+}
+__global__ void k_lcg_uniform(int32_t seed0, int64_t n, double* __restrict__ r, int32_t* __restrict__ seeds) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        // Schrage's step keeps seed_i == seed0 * 16807^i (mod 2^31-1) for any int32 seed0
+        const int64_t s0 = ((int64_t)seed0 % 2147483647ll + 2147483647ll) % 2147483647ll;
+        const uint64_t s = mulmod_m31((uint64_t)s0, powmod_m31(16807ull, (uint64_t)(i + 1)));
+        r[i] = (double)(int32_t)s * 4.656612875e-10;
+        if (seeds) seeds[i] = (int32_t)s;
+    }
+}
+
+
+hipError_t launch_mom(const FieldH* fields, int nfields, const int2* field_tiles, const MomPartial* partials,
+                      MomPartial* field_mom, const double* slot_count, int nslots, double* slot_out, hipStream_t s) {
+    if (nfields <= 0 || nslots <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_mom_field, dim3(nfields), dim3(kBlock), 0, s, fields, field_tiles, partials, field_mom);
+    hipLaunchKernelGGL(k_mom_slot, dim3(nslots), dim3(kBlock), 0, s, fields, nfields, field_mom, slot_count, nslots,
+                       slot_out);
+    return hipGetLastError();
+}
+
+hipError_t launch_spectrum_sum(const FieldH& f, MomPartial* out, int nblocks, hipStream_t s) {
+    hipLaunchKernelGGL(k_spectrum_sum, dim3(nblocks), dim3(kBlock), 0, s, f, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_marginal(double* x, int64_t n, int margi, double mu, double sd, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    const int64_t nb = (n + 255) / 256 < 8192 ? (n + 255) / 256 : 8192;
+    hipLaunchKernelGGL(k_marginal, dim3((unsigned)nb), dim3(256), 0, s, x, n, margi, mu, sd);
+    return hipGetLastError();
+}
+
+hipError_t launch_pou(double* x, int n1, int n2, int n3, int3 lo, int3 hi, hipStream_t s) {
+    const int64_t n = (int64_t)n1 * n2 * n3;
+    if (n <= 0) return hipSuccess;
+    const int64_t nb = (n + 255) / 256 < 8192 ? (n + 255) / 256 : 8192;
+    hipLaunchKernelGGL(k_pou, dim3((unsigned)nb), dim3(256), 0, s, x, n1, n2, n3, lo, hi);
+    return hipGetLastError();
+}
+
+hipError_t launch_sums(const double* rf, int64_t n, int nmc, int nblocks, MomPartial* partial, hipStream_t s) {
+    hipLaunchKernelGGL(k_sums, dim3(nblocks, nmc), dim3(kBlock), 0, s, rf, n, partial);
+    return hipGetLastError();
+}
+
+hipError_t launch_lcg_uniform(int32_t seed0, int64_t n, double* r, int32_t* seeds, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    const int64_t nb = (n + 255) / 256 < 4096 ? (n + 255) / 256 : 4096;
+    hipLaunchKernelGGL(k_lcg_uniform, dim3((unsigned)nb), dim3(256), 0, s, seed0, n, r, seeds);
+    return hipGetLastError();
+}
+
+hipError_t launch_target_corr(int model, double nu, const double* rho, int n, double* out, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_target_corr, dim3((n + 127) / 128), dim3(128), 0, s, model, nu, rho, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_kv(double nu, const double* x, int n, double* out, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_kv, dim3((n + 127) / 128), dim3(128), 0, s, nu, x, n, out);
+    return hipGetLastError();
+}
+
+}  // namespace scarl

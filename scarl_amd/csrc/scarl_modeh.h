@@ -1,0 +1,211 @@
+// scarl_modeh.h -- Hermitian C2R generator passes (device code, templated on precision).
+//
+//   z pass : amp(k) * W(k) generated on the fly (device Philox noise, or host-exported noise)
+//            -> C2C backward along z, into the half-spectrum workspace
+//   y pass : C2C backward along y, in place on the workspace
+//   x pass : C2R along x as a half-length complex FFT, fused with the marginal transform,
+//            the partition-of-unity blend and compensated moments.
+// For a 2D field (n3 = 1) the y pass generates.
+#pragma once
+#include "scarl_fft.h"
+
+namespace scarl {
+
+// amplitude sqrt(S(kappa) * scale) at half-spectrum point (kx, ky, kz)
+template <typename T>
+__device__ __forceinline__ T amplitude_h(const FieldH& f, int kx, int ky, int kz) {
+    const T fx = T(kx), fy = T(signed_freq(ky, f.n[1])), fz = T(signed_freq(kz, f.n[2]));
+    const T a0 = T(f.alpha[0]) * fx, a1 = T(f.alpha[1]) * fy, a2 = T(f.alpha[2]) * fz;
+    const T q = a0 * a0 + a1 * a1 + a2 * a2;
+    if (f.model == CM_GAUSSIAN) {
+        if constexpr (sizeof(T) == 4) return T(f.amp0) * expf(-0.125f * q);
+        else return f.amp0 * exp(-0.125 * q);
+    } else {
+        if constexpr (sizeof(T) == 4) return T(f.amp0) * exp2f(T(f.expo) * log2f(1.0f + q));
+        else return f.amp0 * pow(1.0 + q, f.expo);
+    }
+}
+
+// Hermitian white noise at half-spectrum linear index idx = kx + h1 (ky + n2 kz):
+// off the self-conjugate planes (kx = 0, kx = n1/2) a complex N(0,1) from the point's own counter;
+// on them W(kx,-ky,-kz) = conj W(kx,ky,kz), drawn from the counter of the lower linear index,
+// and the self-conjugate points are real N(0,1).
+template <typename T>
+__device__ __forceinline__ cplx<T> noise_h(const FieldH& f, int kx, int ky, int kz, int64_t idx) {
+    const T rs2 = T(0.70710678118654752440084436210484904);
+    const int n1 = f.n[0], n2 = f.n[1], n3 = f.n[2];
+    const bool selfconj = (kx == 0) || (((n1 & 1) == 0) && kx == n1 / 2);
+    if (!selfconj) {
+        cplx<T> g = gauss_pair<T>(static_cast<uint64_t>(idx), f.seed, f.stream);
+        return {g.x * rs2, g.y * rs2};
+    }
+    const int py = ky ? n2 - ky : 0, pz = kz ? n3 - kz : 0;
+    const int64_t ls = ky + (int64_t)n2 * kz, lp = py + (int64_t)n2 * pz;
+    if (ls == lp) {
+        cplx<T> g = gauss_pair<T>(static_cast<uint64_t>(idx), f.seed, f.stream);
+        return {g.x, T(0)};
+    }
+    const int64_t canon = ls < lp ? ls : lp;
+    cplx<T> g = gauss_pair<T>(static_cast<uint64_t>(kx + (int64_t)f.h1 * canon), f.seed, f.stream);
+    return {g.x * rs2, (ls < lp ? g.y : -g.y) * rs2};
+}
+
+template <typename T>
+__device__ __forceinline__ cplx<T> generate_h(const FieldH& f, int64_t idx) {
+    const int h1 = f.h1, n2 = f.n[1];
+    const int kx = (int)(idx % h1);
+    const int64_t r = idx / h1;
+    const int ky = (int)(r % n2), kz = (int)(r / n2);
+    cplx<T> w;
+    if (f.src_mode == SRC_HOST_NOISE) w = reinterpret_cast<const cplx<T>*>(f.W)[idx];
+    else w = noise_h<T>(f, kx, ky, kz, idx);
+    const T a = amplitude_h<T>(f, kx, ky, kz);
+    return {a * w.x, a * w.y};
+}
+
+// Strided pass along z (axis 2) or y (axis 1), in place on the half-spectrum workspace.
+// GEN: the pass may generate its input (staged through LDS).
+template <typename T, bool GEN>
+__global__ __launch_bounds__(kBlock) void k_h_strided(const FieldH* __restrict__ fields,
+                                                       const Tile* __restrict__ tiles,
+                                                       const PlanDev* __restrict__ plans,
+                                                       const cplx<T>* __restrict__ pool, int axis) {
+    extern __shared__ char smem[];
+    cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem);
+    const Tile tl = tiles[blockIdx.x];
+    const FieldH& f = fields[tl.field];
+    const PlanDev& pl = plans[axis == 2 ? f.plan_z : f.plan_y];
+    const int64_t h1 = f.h1, n2 = f.n[1], n3 = f.n[2];
+    const int64_t A = (axis == 2) ? h1 * n2 : h1;
+    const int64_t B = (axis == 2) ? 0 : h1 * n2;
+    const int64_t S = (axis == 2) ? h1 * n2 : h1;
+    const int64_t npen_total = (axis == 2) ? h1 * n2 : h1 * n3;
+    const int npen = (int)min((int64_t)pl.T, npen_total - tl.pencil0);
+    cplx<T>* ws = reinterpret_cast<cplx<T>*>(f.ws);
+    const bool gen = GEN && (f.first_axis == axis) && (f.src_mode != SRC_WORKSPACE);
+    auto base = [&](int p) -> int64_t {
+        const int64_t P = tl.pencil0 + p;
+        return (P % A) + (P / A) * B;
+    };
+    auto gld = [&](int p, int n) -> cplx<T> {
+        const int64_t idx = base(p) + n * S;
+        if constexpr (GEN) {
+            if (gen) return generate_h<T>(f, idx);
+        }
+        return ws[idx];
+    };
+    auto gst = [&](int p, int n, cplx<T> v) { ws[base(p) + n * S] = v; };
+    pencil_fft<T, LAYOUT_PF, GEN, false>(pl, pool, lds, npen, gld, gst);
+}
+
+// x pass: half-length complex FFT of Z_k = (X_k + X*_{M-k}) + i e^{2 pi i k/n1} (X_k - X*_{M-k})
+// (imaginary parts of X_0 and X_M ignored), x_{2k} = Re z_k, x_{2k+1} = Im z_k; then
+//   no blend: marginal, store, moments;
+//   blend:    value * sqrt(w_PoU); first owner (colour order) stores, later owners add, the last
+//             owner applies the marginal and takes the moments.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_h_xpass(const FieldH* __restrict__ fields, const Tile* __restrict__ tiles,
+                                                     int tile0, const PlanDev* __restrict__ plans,
+                                                     const cplx<T>* __restrict__ pool,
+                                                     MomPartial* __restrict__ partials) {
+    extern __shared__ char smem[];
+    cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem);
+    const int tix = tile0 + blockIdx.x;
+    const Tile tl = tiles[tix];
+    const FieldH& f = fields[tl.field];
+    const PlanDev& pl = plans[f.plan_x];
+    const int n1 = f.n[0], n2 = f.n[1], n3 = f.n[2], h1 = f.h1, M = n1 / 2;
+    const int64_t nrows = (int64_t)n2 * n3;
+    const int npen = (int)min((int64_t)pl.T, nrows - tl.pencil0);
+    const cplx<T>* ws = reinterpret_cast<const cplx<T>*>(f.ws);
+    const cplx<T>* pre = pool + pl.pre_off;
+    T* out = reinterpret_cast<T*>(f.out);
+    const T mu = T(f.mu), sd = T(f.sd);
+    double ms = 0.0, mc = 0.0, ms2 = 0.0, mc2 = 0.0;
+    const bool mom = f.mom_slot >= 0;
+
+    auto gld = [&](int p, int k) -> cplx<T> {
+        const cplx<T>* X = ws + (tl.pencil0 + p) * (int64_t)h1;
+        cplx<T> a = X[k], bb = X[M - k];
+        if (k == 0) {
+            a.y = T(0);
+            bb.y = T(0);
+        }
+        bb.y = -bb.y;
+        const cplx<T> s = a + bb, d = a - bb;
+        const cplx<T> wd = cmul(pre[k], d);
+        return {s.x - wd.y, s.y + wd.x};
+    };
+    auto gst = [&](int p, int k, cplx<T> z) {
+        const int64_t row = tl.pencil0 + p;
+        const int ky = (int)(row % n2), kz = (int)(row / n2);
+        T* o = out + ky * f.out_ld1 + kz * f.out_ld2;
+        if (!f.blend) {
+            const T r0 = apply_margi<T>(z.x, f.margi, mu, sd), r1 = apply_margi<T>(z.y, f.margi, mu, sd);
+            if (mom) {
+                nadd(ms, mc, (double)r0);
+                nadd(ms2, mc2, (double)r0 * (double)r0);
+                nadd(ms, mc, (double)r1);
+                nadd(ms2, mc2, (double)r1 * (double)r1);
+            }
+            if constexpr (sizeof(T) == 4) *reinterpret_cast<float2*>(o + 2 * k) = make_float2(r0, r1);
+            else *reinterpret_cast<double2*>(o + 2 * k) = make_double2(r0, r1);
+            return;
+        }
+        const T wyz = pou_sqrt_weight<T>(ky, n2, f.nov_lo[1], f.nov_hi[1]) *
+                      pou_sqrt_weight<T>(kz, n3, f.nov_lo[2], f.nov_hi[2]);
+        const int myz = (in_overlap(ky, n2, f.nov_lo[1], f.nov_hi[1]) ? 2 : 0) |
+                        (in_overlap(kz, n3, f.nov_lo[2], f.nov_hi[2]) ? 4 : 0);
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            const int i = 2 * k + e;
+            const T w = wyz * pou_sqrt_weight<T>(i, n1, f.nov_lo[0], f.nov_hi[0]);
+            T val = (e ? z.y : z.x) * sqrt(w);
+            const int msk = myz | (in_overlap(i, n1, f.nov_lo[0], f.nov_hi[0]) ? 1 : 0);
+            const bool first = (f.color & msk) == 0;
+            const bool last = (f.color & msk) == msk;
+            if (!first) val += o[i];
+            if (last) {
+                val = apply_margi<T>(val, f.margi, mu, sd);
+                if (mom) {
+                    nadd(ms, mc, (double)val);
+                    nadd(ms2, mc2, (double)val * (double)val);
+                }
+            }
+            o[i] = val;
+        }
+    };
+    pencil_fft<T, LAYOUT_BF, false, true>(pl, pool, lds, npen, gld, gst);
+    if (mom) {
+        __syncthreads();
+        block_reduce_mom(ms, mc, ms2, mc2, reinterpret_cast<double*>(smem), partials + tix);
+    }
+}
+
+}  // namespace scarl
+
+// Instantiate the launch wrappers for one precision T with suffix SFX.
+#define SCARL_MODEH_WRAPPERS(T, SFX)                                                                      \
+    namespace scarl {                                                                                    \
+    hipError_t launch_h_strided_##SFX(int axis, bool gen, const FieldH* fields, const Tile* tiles, int ntiles, \
+                                      const PlanDev* plans, const void* pool, size_t lds_bytes, hipStream_t s) { \
+        if (ntiles <= 0) return hipSuccess;                                                              \
+        const cplx<T>* p = static_cast<const cplx<T>*>(pool);                                            \
+        if (gen)                                                                                         \
+            hipLaunchKernelGGL((k_h_strided<T, true>), dim3(ntiles), dim3(kBlock), lds_bytes, s, fields, tiles, \
+                               plans, p, axis);                                                          \
+        else                                                                                             \
+            hipLaunchKernelGGL((k_h_strided<T, false>), dim3(ntiles), dim3(kBlock), lds_bytes, s, fields, tiles, \
+                               plans, p, axis);                                                          \
+        return hipGetLastError();                                                                        \
+    }                                                                                                    \
+    hipError_t launch_h_xpass_##SFX(const FieldH* fields, const Tile* tiles, int tile0, int ntiles,       \
+                                    const PlanDev* plans, const void* pool, MomPartial* partials,         \
+                                    size_t lds_bytes, hipStream_t s) {                                   \
+        if (ntiles <= 0) return hipSuccess;                                                              \
+        if (lds_bytes < 4 * kBlock * sizeof(double)) lds_bytes = 4 * kBlock * sizeof(double);            \
+        hipLaunchKernelGGL((k_h_xpass<T>), dim3(ntiles), dim3(kBlock), lds_bytes, s, fields, tiles, tile0, plans, \
+                           static_cast<const cplx<T>*>(pool), partials);                                 \
+        return hipGetLastError();                                                                        \
+    }                                                                                                    \
+    }

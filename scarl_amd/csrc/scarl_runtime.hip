@@ -1,0 +1,1308 @@
+// scarl_runtime.hip -- host runtime and C ABI of libscarl_hip.so.
+//
+// Planner (radix split, threads per pencil, pencils per tile, LDS pitch), twiddle pools
+// (computed in long double, rounded once to the kernel precision), field descriptors, tile
+// tables, per-batch hipGraphs, event timing and the extern "C" entry points of scarl_hip.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/scarl_hip.h"
+#include "scarl_device.h"
+#include "scarl_internal.h"
+
+namespace scarl {
+
+// launch wrappers (scarl_kernels.hip)
+hipError_t launch_h_strided_f32(int axis, bool gen, const FieldH* fields, const Tile* tiles, int ntiles,
+                                const PlanDev* plans, const void* pool, size_t lds_bytes, hipStream_t s);
+hipError_t launch_h_strided_f64(int axis, bool gen, const FieldH* fields, const Tile* tiles, int ntiles,
+                                const PlanDev* plans, const void* pool, size_t lds_bytes, hipStream_t s);
+hipError_t launch_h_xpass_f32(const FieldH* fields, const Tile* tiles, int tile0, int ntiles, const PlanDev* plans,
+                              const void* pool, MomPartial* partials, size_t lds_bytes, hipStream_t s);
+hipError_t launch_h_xpass_f64(const FieldH* fields, const Tile* tiles, int tile0, int ntiles, const PlanDev* plans,
+                              const void* pool, MomPartial* partials, size_t lds_bytes, hipStream_t s);
+
+static hipError_t launch_h_strided(bool fp64, int axis, bool gen, const FieldH* fields, const Tile* tiles,
+                                   int ntiles, const PlanDev* plans, const void* pool, size_t lds_bytes,
+                                   hipStream_t s) {
+    return fp64 ? launch_h_strided_f64(axis, gen, fields, tiles, ntiles, plans, pool, lds_bytes, s)
+                : launch_h_strided_f32(axis, gen, fields, tiles, ntiles, plans, pool, lds_bytes, s);
+}
+static hipError_t launch_h_xpass(bool fp64, const FieldH* fields, const Tile* tiles, int tile0, int ntiles,
+                                 const PlanDev* plans, const void* pool, MomPartial* partials, size_t lds_bytes,
+                                 hipStream_t s) {
+    return fp64 ? launch_h_xpass_f64(fields, tiles, tile0, ntiles, plans, pool, partials, lds_bytes, s)
+                : launch_h_xpass_f32(fields, tiles, tile0, ntiles, plans, pool, partials, lds_bytes, s);
+}
+hipError_t launch_mom(const FieldH* fields, int nfields, const int2* field_tiles, const MomPartial* partials,
+                      MomPartial* field_mom, const double* slot_count, int nslots, double* slot_out, hipStream_t s);
+hipError_t launch_spectrum_sum(const FieldH& f, MomPartial* out, int nblocks, hipStream_t s);
+hipError_t launch_r_pass(int axis, const FieldR* f, const PlanDev* plans, const void* pool, int nblocks,
+                         size_t lds_bytes, hipStream_t s);
+hipError_t launch_marginal(double* x, int64_t n, int margi, double mu, double sd, hipStream_t s);
+hipError_t launch_pou(double* x, int n1, int n2, int n3, int3 lo, int3 hi, hipStream_t s);
+hipError_t launch_sums(const double* rf, int64_t n, int nmc, int nblocks, MomPartial* partial, hipStream_t s);
+hipError_t launch_lcg_uniform(int32_t seed0, int64_t n, double* r, int32_t* seeds, hipStream_t s);
+hipError_t launch_target_corr(int model, double nu, const double* rho, int n, double* out, hipStream_t s);
+hipError_t launch_kv(double nu, const double* x, int n, double* out, hipStream_t s);
+
+// ------------------------------------------------------------------------------------------
+// errors
+// ------------------------------------------------------------------------------------------
+static thread_local std::string g_err;
+
+static int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                             \
+    do {                                                                                          \
+        hipError_t e_ = (expr);                                                                   \
+        if (e_ != hipSuccess) return fail(SCARL_E_HIP, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                                          __FILE__, __LINE__);                                    \
+    } while (0)
+
+#define SCARL_TRY(expr)          \
+    do {                         \
+        int rc_ = (expr);        \
+        if (rc_) return rc_;     \
+    } while (0)
+
+// ------------------------------------------------------------------------------------------
+// context: device, stream, timing, scratch buffers for the plain entry points
+// ------------------------------------------------------------------------------------------
+struct TimedLaunch {
+    std::string name;
+    hipEvent_t a, b;
+};
+
+struct Context {
+    bool ready = false;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool timing = false;
+    std::vector<TimedLaunch> pending;
+    std::map<std::string, std::pair<long, double>> totals;
+    std::map<std::string, std::pair<void*, size_t>> scratch;  // named grow-only device buffers
+    int margi = 0;
+    double m_avg = 0.0, m_std = 1.0;
+};
+
+static Context g_ctx;
+
+static int dmalloc(void** p, size_t bytes) {
+    const char* fi = getenv("SCARL_FAULT_INJECT");
+    if (fi && strcmp(fi, "alloc") == 0) return fail(SCARL_E_NOMEM, "device allocation failed (fault injection)");
+    if (bytes == 0) bytes = 16;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(SCARL_E_NOMEM, "hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(e));
+    }
+    return 0;
+}
+
+static int ensure_init() {
+    if (g_ctx.ready) return 0;
+    const char* fi = getenv("SCARL_FAULT_INJECT");
+    if (fi && strcmp(fi, "init") == 0) return fail(SCARL_E_HIP, "no HIP device (fault injection)");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(SCARL_E_HIP, "no HIP device available");
+    if (g_ctx.device >= n) return fail(SCARL_E_INVALID, "device %d out of range (%d devices)", g_ctx.device, n);
+    HIP_TRY(hipSetDevice(g_ctx.device));
+    HIP_TRY(hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking));
+    g_ctx.ready = true;
+    return 0;
+}
+
+static int scratch_get(const std::string& name, size_t bytes, void** p) {
+    auto it = g_ctx.scratch.find(name);
+    if (it != g_ctx.scratch.end() && it->second.second >= bytes) {
+        *p = it->second.first;
+        return 0;
+    }
+    if (it != g_ctx.scratch.end()) {
+        HIP_TRY(hipStreamSynchronize(g_ctx.stream));
+        HIP_TRY(hipFree(it->second.first));
+        g_ctx.scratch.erase(it);
+    }
+    SCARL_TRY(dmalloc(p, bytes));
+    g_ctx.scratch[name] = {*p, bytes};
+    return 0;
+}
+
+// kernel launch bracketed by events when timing is on
+template <class F>
+static int timed(const char* name, F&& launch) {
+    hipEvent_t a = nullptr, b = nullptr;
+    if (g_ctx.timing) {
+        HIP_TRY(hipEventCreate(&a));
+        HIP_TRY(hipEventCreate(&b));
+        HIP_TRY(hipEventRecord(a, g_ctx.stream));
+    }
+    hipError_t e = launch();
+    if (e != hipSuccess) return fail(SCARL_E_HIP, "launch of %s failed: %s", name, hipGetErrorString(e));
+    if (g_ctx.timing) {
+        HIP_TRY(hipEventRecord(b, g_ctx.stream));
+        g_ctx.pending.push_back({name, a, b});
+    }
+    return 0;
+}
+
+static int timing_collect() {
+    if (g_ctx.pending.empty()) return 0;
+    HIP_TRY(hipStreamSynchronize(g_ctx.stream));
+    for (auto& t : g_ctx.pending) {
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, t.a, t.b));
+        auto& tot = g_ctx.totals[t.name];
+        tot.first += 1;
+        tot.second += ms;
+        hipEventDestroy(t.a);
+        hipEventDestroy(t.b);
+    }
+    g_ctx.pending.clear();
+    return 0;
+}
+
+// ------------------------------------------------------------------------------------------
+// planner and twiddle pools
+// ------------------------------------------------------------------------------------------
+static const long double kPiL = 3.14159265358979323846264338327950288L;
+
+// exp(+2 pi i num/den) in long double
+static void cis_ld(long long num, long long den, long double& c, long double& s) {
+    num %= den;
+    if (num < 0) num += den;
+    const long double a = 2.0L * kPiL * (long double)num / (long double)den;
+    c = cosl(a);
+    s = sinl(a);
+}
+
+struct PlanSet {
+    std::vector<PlanDev> plans;
+    std::vector<long double> pool;  // interleaved re/im, long double; rounded once at upload
+    std::map<std::string, int> index;
+    PlanDev* d_plans = nullptr;
+    void* d_pool = nullptr;
+    bool fp64 = false;
+    size_t max_lds = 0;
+
+    ~PlanSet() {
+        if (d_plans) hipFree(d_plans);
+        if (d_pool) hipFree(d_pool);
+    }
+
+    int pool_push(long double c, long double s) {
+        const int off = (int)(pool.size() / 2);
+        pool.push_back(c);
+        pool.push_back(s);
+        return off;
+    }
+
+    // radix split: powers of two in chunks no larger than the register capacity, then the odd
+    // primes 3, 5, 7, 11, 13; anything else -> direct DFT.
+    static bool factor(int N, int ecap, std::vector<int>& rad) {
+        rad.clear();
+        int a = 0, n = N;
+        while (n % 2 == 0) { n /= 2; ++a; }
+        std::vector<int> odd;
+        for (int p : {3, 5, 7, 11, 13})
+            while (n % p == 0) { n /= p; odd.push_back(p); }
+        if (n != 1) return false;
+        const int maxr = ecap >= 32 ? 32 : 16;
+        const int lmax = maxr == 32 ? 5 : 4;
+        // split a into chunks of at most lmax bits, as evenly as possible, largest first
+        if (a > 0) {
+            const int k = (a + lmax - 1) / lmax;
+            std::vector<int> bits(k, a / k);
+            for (int i = 0; i < a % k; ++i) bits[i] += 1;
+            for (int b : bits) rad.push_back(1 << b);
+        }
+        for (int p : odd) rad.push_back(p);
+        return true;
+    }
+
+    // kind: 0 none, 1 C2R post-twiddle (exp(2 pi i k/(2N)), k<N), 2 DCT pre-twiddle (exp(i pi j/(2N)))
+    int get(int N, int layout, int kind) {
+        char key[64];
+        snprintf(key, sizeof key, "%d/%d/%d", N, layout, kind);
+        auto it = index.find(key);
+        if (it != index.end()) return it->second;
+        const int ecap = fp64 ? kEcap64 : kEcap32;
+        const int csz = fp64 ? 16 : 8;
+        PlanDev pl;
+        memset(&pl, 0, sizeof pl);
+        pl.N = N;
+        pl.direct_off = -1;
+        pl.pre_off = -1;
+        std::vector<int> rad;
+        if (N >= 2 && factor(N, ecap, rad)) {
+            pl.nst = (int)rad.size();
+            int Ns = 1;
+            int tpp = 1;
+            for (int s = 0; s < pl.nst; ++s) {
+                const int R = rad[s];
+                pl.R[s] = R;
+                pl.Ns[s] = Ns;
+                pl.tw_off[s] = 0;
+                if (Ns > 1) {
+                    pl.tw_off[s] = (int)(pool.size() / 2);
+                    for (int k = 0; k < Ns; ++k)
+                        for (int r = 1; r < R; ++r) {
+                            long double c, sn;
+                            cis_ld((long long)k * r, (long long)Ns * R, c, sn);
+                            pool_push(c, sn);
+                        }
+                }
+                const int nb = N / R, bmax = std::max(1, ecap / R);
+                tpp = std::max(tpp, (nb + bmax - 1) / bmax);
+                Ns *= R;
+            }
+            pl.tpp = tpp;
+        } else {
+            pl.nst = 0;  // direct DFT (N == 1 or a prime factor above 13)
+            if (N > 4096) return -fail(SCARL_E_UNSUPPORTED, "length %d has a prime factor above 13 and exceeds 4096", N);
+            pl.direct_off = (int)(pool.size() / 2);
+            for (int t = 0; t < N; ++t) {
+                long double c, sn;
+                cis_ld(t, N, c, sn);
+                pool_push(c, sn);
+            }
+            pl.tpp = std::min(N, 64);
+        }
+        if (pl.tpp > kBlock) return -fail(SCARL_E_UNSUPPORTED, "transform length %d too large for one tile", N);
+        pl.T = kBlock / pl.tpp;
+        pl.ldp = N + N / 16 + 1;
+        // keep the LDS tile within 64 KiB so two tiles fit on a CU.  Every plan gets an LDS
+        // tile, because passes that stage their input or output use it even for one stage.
+        const size_t per = (size_t)(layout == LAYOUT_PF ? N : pl.ldp) * csz;
+        const int Tmax = (int)std::max<size_t>(1, (64 * 1024) / per);
+        pl.T = std::min(pl.T, Tmax);
+        pl.lds_elems = (int)(pl.T * (layout == LAYOUT_PF ? N : pl.ldp));
+        if (kind == 1) {
+            pl.pre_off = (int)(pool.size() / 2);
+            for (int k = 0; k < N; ++k) {
+                long double c, sn;
+                cis_ld(k, 2LL * N, c, sn);
+                pool_push(c, sn);
+            }
+        } else if (kind == 2) {
+            pl.pre_off = (int)(pool.size() / 2);
+            for (int j = 0; j < N; ++j) {
+                long double c, sn;
+                cis_ld(j, 4LL * N, c, sn);  // exp(i pi j/(2N)) = exp(2 pi i j/(4N))
+                pool_push(c, sn);
+            }
+        }
+        const size_t lds = (size_t)pl.lds_elems * csz;
+        max_lds = std::max(max_lds, lds);
+        plans.push_back(pl);
+        const int id = (int)plans.size() - 1;
+        index[key] = id;
+        return id;
+    }
+
+    int upload() {
+        if (d_plans) { hipFree(d_plans); d_plans = nullptr; }
+        if (d_pool) { hipFree(d_pool); d_pool = nullptr; }
+        SCARL_TRY(dmalloc((void**)&d_plans, plans.size() * sizeof(PlanDev)));
+        HIP_TRY(hipMemcpy(d_plans, plans.data(), plans.size() * sizeof(PlanDev), hipMemcpyHostToDevice));
+        const size_t nc = std::max<size_t>(1, pool.size() / 2);
+        if (fp64) {
+            std::vector<double> h(2 * nc, 0.0);
+            for (size_t i = 0; i < pool.size(); ++i) h[i] = (double)pool[i];
+            SCARL_TRY(dmalloc(&d_pool, h.size() * sizeof(double)));
+            HIP_TRY(hipMemcpy(d_pool, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+        } else {
+            std::vector<float> h(2 * nc, 0.f);
+            for (size_t i = 0; i < pool.size(); ++i) h[i] = (float)pool[i];
+            SCARL_TRY(dmalloc(&d_pool, h.size() * sizeof(float)));
+            HIP_TRY(hipMemcpy(d_pool, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
+        }
+        return 0;
+    }
+};
+
+static int check_lds(size_t bytes) {
+    if (bytes > 160 * 1024) return fail(SCARL_E_UNSUPPORTED, "tile needs %zu bytes of LDS", bytes);
+    return 0;
+}
+
+// ------------------------------------------------------------------------------------------
+// marginal parameters (SRC_F/main_ScaRL.f90:1367-1387)
+// ------------------------------------------------------------------------------------------
+static int marginal_params(int margi, double avg, double std_dev, double* mu, double* sd) {
+    double var;
+    if (margi == FOM_NONE) {
+        *mu = 0.0;
+        *sd = 1.0;
+        return 0;
+    }
+    if (margi == FOM_GAUSSIAN) {
+        var = std_dev * std_dev;
+        *mu = avg;
+    } else if (margi == FOM_LOGNORMAL) {
+        if (avg <= 0.0) return fail(SCARL_E_INVALID, "ERROR - when using lognormal fieldAvg should be a positive number");
+        var = log(1.0 + (std_dev / avg) * (std_dev / avg));
+        *mu = log(avg) - var / 2.0;
+    } else {
+        return fail(SCARL_E_INVALID, "First-order marginal not detected (margiFirst = %d)", margi);
+    }
+    *sd = sqrt(var);
+    return 0;
+}
+
+// ------------------------------------------------------------------------------------------
+// batch of mode H fields
+// ------------------------------------------------------------------------------------------
+struct FieldHost {
+    scarl_field_desc d;
+    FieldH f;
+    double spec_sum = 0.0;
+    int64_t out_elems = 0;
+    void* noise_buf = nullptr;
+};
+
+}  // namespace scarl
+
+struct scarl_batch {
+    bool fp64 = false;
+    bool global = false;
+    bool any_2d = false;  // some field is 2D: its y pass generates the spectrum
+    int gdim[3] = {0, 0, 0};
+    std::vector<scarl::FieldHost> fields;
+    scarl::PlanSet ps;
+    std::vector<scarl::Tile> tz, ty, tx;     // tiles of the z, y and x passes
+    std::vector<int> x_color_begin;          // x tiles grouped by colour (blend): 9 offsets
+    std::vector<int2> field_xtiles;          // per-field [begin, end) in tx
+    scarl::FieldH* d_fields = nullptr;
+    scarl::Tile *d_tz = nullptr, *d_ty = nullptr, *d_tx = nullptr;
+    int2* d_field_xtiles = nullptr;
+    scarl::MomPartial* d_partials = nullptr;
+    scarl::MomPartial* d_field_mom = nullptr;
+    double* d_slot_count = nullptr;
+    double* d_slot_out = nullptr;
+    int nslots = 0;
+    void* d_ws = nullptr;       // all half-spectrum workspaces
+    void* d_out = nullptr;      // all outputs (or the global field)
+    int64_t bytes = 0;
+    size_t lds_z = 0, lds_y = 0, lds_x = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+
+    ~scarl_batch() {
+        if (gexec) hipGraphExecDestroy(gexec);
+        if (graph) hipGraphDestroy(graph);
+        for (auto& f : fields)
+            if (f.noise_buf) hipFree(f.noise_buf);
+        void* ptrs[] = {d_fields, d_tz, d_ty, d_tx, d_field_xtiles, d_partials, d_field_mom, d_slot_count, d_slot_out,
+                        d_ws, d_out};
+        for (void* p : ptrs)
+            if (p) hipFree(p);
+    }
+};
+
+namespace scarl {
+
+static int validate_desc(const scarl_field_desc& d) {
+    for (int a = 0; a < 3; ++a)
+        if (d.n[a] < 1) return fail(SCARL_E_INVALID, "extent n[%d] = %d must be >= 1", a, d.n[a]);
+    if (d.n[0] % 2) return fail(SCARL_E_UNSUPPORTED, "Hermitian mode needs an even x extent (n[0] = %d)", d.n[0]);
+    if (d.n[0] < 2) return fail(SCARL_E_INVALID, "x extent must be >= 2");
+    if (d.n[2] > 1 && d.n[1] < 2) return fail(SCARL_E_INVALID, "a 3D field needs n[1] >= 2");
+    if (d.n[1] < 2) return fail(SCARL_E_UNSUPPORTED, "1D fields are not supported (n[1] = 1)");
+    if (d.model != CM_GAUSSIAN && d.model != CM_VON_KARMAN && d.model != CM_EXPONENTIAL)
+        return fail(SCARL_E_INVALID, "unknown correlation model %d", d.model);
+    for (int a = 0; a < 3; ++a) {
+        if (!(d.dx[a] > 0.0)) return fail(SCARL_E_INVALID, "dx[%d] must be positive", a);
+        if (!(d.corrL[a] > 0.0)) return fail(SCARL_E_INVALID, "corrL[%d] must be positive", a);
+    }
+    if (d.model == CM_VON_KARMAN && !(d.nu > 0.0)) return fail(SCARL_E_INVALID, "von Karman needs nu > 0");
+    if (!(d.sigma >= 0.0)) return fail(SCARL_E_INVALID, "sigma must be >= 0");
+    return 0;
+}
+
+// Build the FieldH of one descriptor (spectrum constants; amp0 is finished after the FP64
+// spectrum sum).  Returns 0 or an error.
+static int build_field(scarl_batch* b, FieldHost& fh) {
+    const scarl_field_desc& d = fh.d;
+    FieldH& f = fh.f;
+    memset(&f, 0, sizeof f);
+    for (int a = 0; a < 3; ++a) f.n[a] = d.n[a];
+    f.h1 = d.n[0] / 2 + 1;
+    const int dim = d.n[2] > 1 ? 3 : 2;
+    f.first_axis = (dim == 3) ? 2 : 1;
+    f.src_mode = SRC_DEVICE_NOISE;
+    f.model = d.model;
+    for (int a = 0; a < 3; ++a) f.alpha[a] = (a < dim) ? 2.0 * M_PI * d.corrL[a] / (d.n[a] * d.dx[a]) : 0.0;
+    const double nu = (d.model == CM_EXPONENTIAL) ? 0.5 : d.nu;
+    f.expo = -(nu + 0.5 * dim) / 2.0;
+    f.seed = d.seed;
+    f.stream = d.stream;
+    f.margi = d.margi;
+    SCARL_TRY(marginal_params(d.margi, d.avg, d.std_dev, &f.mu, &f.sd));
+    f.mom_slot = -1;
+    // plans
+    const int kz = (dim == 3) ? b->ps.get(d.n[2], LAYOUT_PF, 0) : 0;
+    if (kz < 0) return -kz;
+    const int ky = b->ps.get(d.n[1], LAYOUT_PF, 0);
+    if (ky < 0) return -ky;
+    const int kx = b->ps.get(d.n[0] / 2, LAYOUT_BF, 1);
+    if (kx < 0) return -kx;
+    f.plan_z = kz;
+    f.plan_y = ky;
+    f.plan_x = kx;
+    return 0;
+}
+
+static double spectrum_prefactor(const scarl_field_desc& d, int dim) {
+    // c such that S = c * g(q): Gaussian prod L/(2 sqrt pi); Matern Gamma(e)/(Gamma(nu) pi^(dim/2)) prod L
+    double pL = 1.0;
+    for (int a = 0; a < dim; ++a) pL *= d.corrL[a];
+    if (d.model == CM_GAUSSIAN) return pL / pow(2.0 * sqrt(M_PI), dim);
+    const double nu = (d.model == CM_EXPONENTIAL) ? 0.5 : d.nu;
+    const double e = nu + 0.5 * dim;
+    return tgamma(e) / (tgamma(nu) * pow(M_PI, 0.5 * dim)) * pL;
+}
+
+// FP64 device reduction of the spectrum over every field; fills amp0.
+static int finish_amplitudes(scarl_batch* b) {
+    int maxblocks = 1;
+    for (auto& fh : b->fields) {
+        const int64_t cols = (int64_t)fh.f.h1 * fh.f.n[1];
+        maxblocks = std::max<int>(maxblocks, (int)((cols + kBlock - 1) / kBlock));
+    }
+    MomPartial* d_part = nullptr;
+    SCARL_TRY(dmalloc((void**)&d_part, sizeof(MomPartial) * maxblocks));
+    std::vector<MomPartial> h(maxblocks);
+    std::map<std::string, double> cache;
+    int rc = 0;
+    for (auto& fh : b->fields) {
+        const FieldH& f = fh.f;
+        char key[256];
+        snprintf(key, sizeof key, "%d %d %d %d %.17g %.17g %.17g %.17g", f.n[0], f.n[1], f.n[2], f.model, f.alpha[0],
+                 f.alpha[1], f.alpha[2], f.expo);
+        auto it = cache.find(key);
+        double G;
+        if (it != cache.end()) {
+            G = it->second;
+        } else {
+            const int64_t cols = (int64_t)f.h1 * f.n[1];
+            const int nblocks = (int)((cols + kBlock - 1) / kBlock);
+            hipError_t e = launch_spectrum_sum(f, d_part, nblocks, g_ctx.stream);
+            if (e == hipSuccess) e = hipMemcpyAsync(h.data(), d_part, sizeof(MomPartial) * nblocks,
+                                                    hipMemcpyDeviceToHost, g_ctx.stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(g_ctx.stream);
+            if (e != hipSuccess) {
+                rc = fail(SCARL_E_HIP, "spectrum sum: %s", hipGetErrorString(e));
+                break;
+            }
+            // Neumaier combine of the block partials on the host
+            double s = 0.0, c = 0.0;
+            for (int i = 0; i < nblocks; ++i) {
+                const double v = h[i].s;
+                const double t = s + v;
+                c += (fabs(s) >= fabs(v)) ? (s - t) + v : (v - t) + s;
+                s = t;
+                c += h[i].c;
+            }
+            G = s + c;
+            cache[key] = G;
+        }
+        const int dim = f.n[2] > 1 ? 3 : 2;
+        const double cpre = spectrum_prefactor(fh.d, dim);
+        fh.spec_sum = cpre * G;  // sum of S over the full discrete spectrum
+        if (fh.d.normalize) {
+            fh.f.amp0 = fh.d.sigma / sqrt(G);
+        } else {
+            double dk = 1.0;
+            for (int a = 0; a < dim; ++a) dk *= 2.0 * M_PI / (fh.d.n[a] * fh.d.dx[a]);
+            fh.f.amp0 = fh.d.sigma * sqrt(cpre * dk);
+        }
+    }
+    hipFree(d_part);
+    return rc;
+}
+
+static void make_tiles(scarl_batch* b) {
+    b->tz.clear();
+    b->ty.clear();
+    b->tx.clear();
+    b->field_xtiles.assign(b->fields.size(), make_int2(0, 0));
+    for (size_t i = 0; i < b->fields.size(); ++i) {
+        const FieldH& f = b->fields[i].f;
+        const int64_t h1 = f.h1, n2 = f.n[1], n3 = f.n[2];
+        if (n3 > 1) {
+            const int T = b->ps.plans[f.plan_z].T;
+            for (int64_t p = 0; p < h1 * n2; p += T) b->tz.push_back({(int)i, (int)p});
+        }
+        {
+            const int T = b->ps.plans[f.plan_y].T;
+            for (int64_t p = 0; p < h1 * n3; p += T) b->ty.push_back({(int)i, (int)p});
+        }
+    }
+    // x tiles, grouped by colour so the blend can run colour by colour
+    b->x_color_begin.assign(9, 0);
+    for (int c = 0; c < 8; ++c) {
+        b->x_color_begin[c] = (int)b->tx.size();
+        for (size_t i = 0; i < b->fields.size(); ++i) {
+            const FieldH& f = b->fields[i].f;
+            if ((f.color & 7) != c) continue;
+            const int T = b->ps.plans[f.plan_x].T;
+            const int64_t rows = (int64_t)f.n[1] * f.n[2];
+            b->field_xtiles[i].x = (int)b->tx.size();
+            for (int64_t p = 0; p < rows; p += T) b->tx.push_back({(int)i, (int)p});
+            b->field_xtiles[i].y = (int)b->tx.size();
+        }
+    }
+    b->x_color_begin[8] = (int)b->tx.size();
+}
+
+template <class V>
+static int upload_vec(const V& v, void** d) {
+    using E = typename V::value_type;
+    SCARL_TRY(dmalloc(d, std::max<size_t>(1, v.size()) * sizeof(E)));
+    if (!v.empty()) HIP_TRY(hipMemcpy(*d, v.data(), v.size() * sizeof(E), hipMemcpyHostToDevice));
+    return 0;
+}
+
+// Allocate workspaces/outputs, tile tables, moments; upload descriptors.
+static int finalize_batch(scarl_batch* b, const std::vector<double>& slot_count) {
+    const size_t rsz = b->fp64 ? 8 : 4;
+    int64_t ws_elems = 0, out_elems = 0;
+    for (auto& fh : b->fields) ws_elems += (int64_t)fh.f.h1 * fh.f.n[1] * fh.f.n[2];
+    b->any_2d = false;
+    for (auto& fh : b->fields) b->any_2d = b->any_2d || fh.f.n[2] == 1;
+    if (!b->global)
+        for (auto& fh : b->fields) out_elems += (int64_t)fh.f.n[0] * fh.f.n[1] * fh.f.n[2];
+    else
+        out_elems = (int64_t)b->gdim[0] * b->gdim[1] * b->gdim[2];
+    SCARL_TRY(dmalloc(&b->d_ws, (size_t)ws_elems * 2 * rsz));
+    SCARL_TRY(dmalloc(&b->d_out, (size_t)out_elems * rsz));
+    b->bytes = ws_elems * 2 * rsz + out_elems * rsz;
+    int64_t wo = 0, oo = 0;
+    for (auto& fh : b->fields) {
+        FieldH& f = fh.f;
+        f.ws = static_cast<char*>(b->d_ws) + wo * 2 * rsz;
+        wo += (int64_t)f.h1 * f.n[1] * f.n[2];
+        if (!b->global) {
+            f.out = static_cast<char*>(b->d_out) + oo * rsz;
+            f.out_ld1 = f.n[0];
+            f.out_ld2 = (int64_t)f.n[0] * f.n[1];
+            fh.out_elems = (int64_t)f.n[0] * f.n[1] * f.n[2];
+            oo += fh.out_elems;
+        }
+    }
+    SCARL_TRY(b->ps.upload());
+    make_tiles(b);
+    b->lds_z = b->lds_y = b->lds_x = 0;
+    const size_t csz = 2 * rsz;
+    for (auto& fh : b->fields) {
+        const FieldH& f = fh.f;
+        if (f.n[2] > 1) b->lds_z = std::max(b->lds_z, (size_t)b->ps.plans[f.plan_z].lds_elems * csz);
+        b->lds_y = std::max(b->lds_y, (size_t)b->ps.plans[f.plan_y].lds_elems * csz);
+        b->lds_x = std::max(b->lds_x, (size_t)b->ps.plans[f.plan_x].lds_elems * csz);
+    }
+    b->lds_x = std::max(b->lds_x, (size_t)(4 * kBlock * sizeof(double)));
+    SCARL_TRY(check_lds(b->lds_z));
+    SCARL_TRY(check_lds(b->lds_y));
+    SCARL_TRY(check_lds(b->lds_x));
+    std::vector<FieldH> hf;
+    for (auto& fh : b->fields) hf.push_back(fh.f);
+    SCARL_TRY(upload_vec(hf, (void**)&b->d_fields));
+    SCARL_TRY(upload_vec(b->tz, (void**)&b->d_tz));
+    SCARL_TRY(upload_vec(b->ty, (void**)&b->d_ty));
+    SCARL_TRY(upload_vec(b->tx, (void**)&b->d_tx));
+    SCARL_TRY(upload_vec(b->field_xtiles, (void**)&b->d_field_xtiles));
+    SCARL_TRY(dmalloc((void**)&b->d_partials, std::max<size_t>(1, b->tx.size()) * sizeof(MomPartial)));
+    SCARL_TRY(dmalloc((void**)&b->d_field_mom, std::max<size_t>(1, b->fields.size()) * sizeof(MomPartial)));
+    b->nslots = (int)slot_count.size();
+    SCARL_TRY(upload_vec(slot_count, (void**)&b->d_slot_count));
+    SCARL_TRY(dmalloc((void**)&b->d_slot_out, std::max<size_t>(1, slot_count.size()) * 5 * sizeof(double)));
+    return 0;
+}
+
+static int enqueue_batch(scarl_batch* b) {
+    hipStream_t s = g_ctx.stream;
+    const bool f64 = b->fp64;
+    SCARL_TRY(timed("h_zpass", [&] {
+        return launch_h_strided(f64, 2, true, b->d_fields, b->d_tz, (int)b->tz.size(), b->ps.d_plans,
+                                b->ps.d_pool, b->lds_z, s);
+    }));
+    SCARL_TRY(timed("h_ypass", [&] {
+        return launch_h_strided(f64, 1, b->any_2d, b->d_fields, b->d_ty, (int)b->ty.size(), b->ps.d_plans,
+                                b->ps.d_pool, b->lds_y, s);
+    }));
+    for (int c = 0; c < 8; ++c) {
+        const int t0 = b->x_color_begin[c], t1 = b->x_color_begin[c + 1];
+        if (t1 <= t0) continue;
+        SCARL_TRY(timed("h_xpass", [&] {
+            return launch_h_xpass(f64, b->d_fields, b->d_tx, t0, t1 - t0, b->ps.d_plans, b->ps.d_pool,
+                                  b->d_partials, b->lds_x, s);
+        }));
+    }
+    if (b->nslots > 0) {
+        SCARL_TRY(timed("moments", [&] {
+            return launch_mom(b->d_fields, (int)b->fields.size(), b->d_field_xtiles, b->d_partials, b->d_field_mom,
+                              b->d_slot_count, b->nslots, b->d_slot_out, s);
+        }));
+    }
+    return 0;
+}
+
+static int upload_fields(scarl_batch* b) {
+    std::vector<FieldH> hf;
+    for (auto& fh : b->fields) hf.push_back(fh.f);
+    HIP_TRY(hipMemcpy(b->d_fields, hf.data(), hf.size() * sizeof(FieldH), hipMemcpyHostToDevice));
+    return 0;
+}
+
+static void drop_graph(scarl_batch* b) {
+    if (b->gexec) hipGraphExecDestroy(b->gexec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    b->gexec = nullptr;
+    b->graph = nullptr;
+}
+
+// ------------------------------------------------------------------------------------------
+// mode R plain path (one field per call, FP64)
+// ------------------------------------------------------------------------------------------
+struct ModeRCache {
+    int n[3] = {0, 0, 0}, m[3] = {0, 0, 0};
+    std::unique_ptr<PlanSet> ps;
+    int plan[3] = {0, 0, 0};
+    size_t lds[3] = {0, 0, 0};
+    FieldR* d_field = nullptr;
+};
+static ModeRCache g_r;
+
+static int mode_r_prepare(const int32_t n[3], const int32_t m[3]) {
+    if (g_r.ps && memcmp(g_r.n, n, sizeof g_r.n) == 0 && memcmp(g_r.m, m, sizeof g_r.m) == 0) return 0;
+    std::unique_ptr<PlanSet> ps(new PlanSet());
+    ps->fp64 = true;
+    int plan[3];
+    for (int a = 0; a < 3; ++a) {
+        plan[a] = ps->get(n[a], a == 0 ? LAYOUT_BF : LAYOUT_PF, 2);
+        if (plan[a] < 0) return -plan[a];
+    }
+    SCARL_TRY(ps->upload());
+    for (int a = 0; a < 3; ++a) {
+        g_r.lds[a] = (size_t)ps->plans[plan[a]].lds_elems * 16;
+        SCARL_TRY(check_lds(g_r.lds[a]));
+        g_r.plan[a] = plan[a];
+        g_r.n[a] = n[a];
+        g_r.m[a] = m[a];
+    }
+    if (!g_r.d_field) SCARL_TRY(dmalloc((void**)&g_r.d_field, sizeof(FieldR)));
+    g_r.ps = std::move(ps);
+    return 0;
+}
+
+// d_out, d_phi device pointers; m = octant support
+static int mode_r_run(double* d_out, const int32_t n[3], const int32_t m[3], const double xRange[3],
+                      const double corrL[3], int32_t corrMod, uint64_t seed, const double* d_phi, int32_t flags) {
+    if (corrMod != CM_GAUSSIAN)
+        return fail(SCARL_E_UNSUPPORTED, "gen_std_gauss_FFT implements corrMod = cm_GAUSSIAN only (got %d)", corrMod);
+    for (int a = 0; a < 3; ++a) {
+        if (n[a] < 1) return fail(SCARL_E_INVALID, "Np(%d) = %d must be >= 1", a + 1, n[a]);
+        if (!(corrL[a] > 0.0)) return fail(SCARL_E_INVALID, "corrL(%d) must be positive", a + 1);
+    }
+    SCARL_TRY(mode_r_prepare(n, m));
+    FieldR f;
+    memset(&f, 0, sizeof f);
+    for (int a = 0; a < 3; ++a) {
+        f.n[a] = n[a];
+        f.m[a] = m[a];
+        f.plan[a] = g_r.plan[a];
+        // SRC_F/fftw_ScaRL.f90:300-302
+        const double delta_x = xRange[a] / (double)(n[a] - 1);
+        const double k_max = 1.0 / delta_x;
+        f.dk[a] = k_max / (double)(n[a] - 1);
+        f.corrL[a] = corrL[a];
+    }
+    f.src_mode = d_phi ? SRC_HOST_NOISE : SRC_DEVICE_NOISE;
+    f.phi = d_phi;
+    f.seed = seed;
+    f.stream = 0;
+    f.margi = (flags & 1) ? g_ctx.margi : FOM_NONE;
+    SCARL_TRY(marginal_params(f.margi, g_ctx.m_avg, g_ctx.m_std, &f.mu, &f.sd));
+    const int64_t t1 = (int64_t)n[0] * m[1] * m[2], t2 = (int64_t)n[0] * n[1] * m[2];
+    void *p1, *p2;
+    SCARL_TRY(scratch_get("r_tmp1", (size_t)std::max<int64_t>(1, t1) * 8, &p1));
+    SCARL_TRY(scratch_get("r_tmp2", (size_t)std::max<int64_t>(1, t2) * 8, &p2));
+    f.tmp1 = static_cast<double*>(p1);
+    f.tmp2 = static_cast<double*>(p2);
+    f.out = d_out;
+    HIP_TRY(hipMemcpyAsync(g_r.d_field, &f, sizeof f, hipMemcpyHostToDevice, g_ctx.stream));
+    const int64_t npen_real[3] = {(int64_t)m[1] * m[2], (int64_t)n[0] * m[2], (int64_t)n[0] * n[1]};
+    static const char* names[3] = {"r_xpass", "r_ypass", "r_zpass"};
+    for (int a = 0; a < 3; ++a) {
+        const PlanDev& pl = g_r.ps->plans[g_r.plan[a]];
+        const int64_t ncplx = (npen_real[a] + 1) / 2;
+        const int64_t nblocks = (ncplx + pl.T - 1) / pl.T;
+        if (nblocks > 0x7fffffff) return fail(SCARL_E_UNSUPPORTED, "grid too large");
+        if (a == 0 && (m[1] == 0 || m[2] == 0)) {
+            // empty octant: x pass has no pencils; tmp1 content is irrelevant (m2 or m3 = 0)
+            continue;
+        }
+        SCARL_TRY(timed(names[a], [&] {
+            return launch_r_pass(a, g_r.d_field, g_r.ps->d_plans, g_r.ps->d_pool, (int)nblocks, g_r.lds[a],
+                                 g_ctx.stream);
+        }));
+    }
+    return 0;
+}
+
+}  // namespace scarl
+
+using namespace scarl;
+
+// ==========================================================================================
+// extern "C" entry points
+// ==========================================================================================
+extern "C" {
+
+int scarl_hip_init(int32_t device) {
+    if (g_ctx.ready && device != g_ctx.device) return fail(SCARL_E_INVALID, "already initialised on device %d", g_ctx.device);
+    g_ctx.device = device;
+    return ensure_init();
+}
+
+int scarl_hip_finalize(void) {
+    if (!g_ctx.ready) return 0;
+    hipStreamSynchronize(g_ctx.stream);
+    for (auto& kv : g_ctx.scratch) hipFree(kv.second.first);
+    g_ctx.scratch.clear();
+    for (auto& t : g_ctx.pending) {
+        hipEventDestroy(t.a);
+        hipEventDestroy(t.b);
+    }
+    g_ctx.pending.clear();
+    g_r.ps.reset();
+    if (g_r.d_field) hipFree(g_r.d_field);
+    g_r.d_field = nullptr;
+    memset(g_r.n, 0, sizeof g_r.n);
+    hipStreamDestroy(g_ctx.stream);
+    g_ctx.stream = nullptr;
+    g_ctx.ready = false;
+    return 0;
+}
+
+int scarl_hip_last_error(char* buf, int32_t len) {
+    if (!buf || len <= 0) return SCARL_E_INVALID;
+    snprintf(buf, (size_t)len, "%s", g_err.c_str());
+    return 0;
+}
+
+int scarl_hip_set_timing(int32_t on) {
+    SCARL_TRY(ensure_init());
+    g_ctx.timing = on != 0;
+    return 0;
+}
+
+int scarl_hip_timing_report(char* buf, int32_t len) {
+    SCARL_TRY(ensure_init());
+    SCARL_TRY(timing_collect());
+    std::string s;
+    char line[256];
+    for (auto& kv : g_ctx.totals) {
+        snprintf(line, sizeof line, "%s %ld %.6f\n", kv.first.c_str(), kv.second.first, kv.second.second);
+        s += line;
+    }
+    if (!buf || len <= 0) return SCARL_E_INVALID;
+    snprintf(buf, (size_t)len, "%s", s.c_str());
+    return 0;
+}
+
+int scarl_hip_timing_reset(void) {
+    SCARL_TRY(ensure_init());
+    SCARL_TRY(timing_collect());
+    g_ctx.totals.clear();
+    return 0;
+}
+
+int scarl_hip_get_stream(void** stream) {
+    SCARL_TRY(ensure_init());
+    if (!stream) return fail(SCARL_E_INVALID, "null stream pointer");
+    *stream = (void*)g_ctx.stream;
+    return 0;
+}
+
+int scarl_hip_set_marginal(int32_t margiFirst, double avg, double std_dev) {
+    double mu, sd;
+    SCARL_TRY(marginal_params(margiFirst, avg, std_dev, &mu, &sd));
+    g_ctx.margi = margiFirst;
+    g_ctx.m_avg = avg;
+    g_ctx.m_std = std_dev;
+    return 0;
+}
+
+// ---- fftw_ScaRL --------------------------------------------------------------------------
+static int gen_r_host(double* h_out, const int32_t n[3], const int32_t m[3], const double xRange[3],
+                      const double corrL[3], int32_t corrMod, int32_t seed, int32_t rank, const double* h_phi,
+                      int32_t flags) {
+    SCARL_TRY(ensure_init());
+    if (!h_out) return fail(SCARL_E_INVALID, "null output");
+    const int64_t N = (int64_t)n[0] * n[1] * n[2];
+    const int64_t P = (int64_t)m[0] * m[1] * m[2];
+    void *d_out, *d_phi = nullptr;
+    SCARL_TRY(scratch_get("r_out", (size_t)std::max<int64_t>(1, N) * 8, &d_out));
+    if (h_phi) {
+        SCARL_TRY(scratch_get("r_phi", (size_t)std::max<int64_t>(1, P) * 8, &d_phi));
+        if (P > 0) HIP_TRY(hipMemcpyAsync(d_phi, h_phi, (size_t)P * 8, hipMemcpyHostToDevice, g_ctx.stream));
+    }
+    const uint64_t key = (uint64_t)(int64_t)((int64_t)seed + rank);
+    SCARL_TRY(mode_r_run((double*)d_out, n, m, xRange, corrL, corrMod, key, (const double*)d_phi, flags));
+    HIP_TRY(hipMemcpyAsync(h_out, d_out, (size_t)N * 8, hipMemcpyDeviceToHost, g_ctx.stream));
+    HIP_TRY(hipStreamSynchronize(g_ctx.stream));
+    return 0;
+}
+
+int scarl_hip_gen_std_gauss_fft(double* h_out, const int32_t Np[3], const double xRange[3], const double corrL[3],
+                                int32_t corrMod, int32_t seed, int32_t rank, const double* h_phi, int32_t flags) {
+    if (!Np || !xRange || !corrL) return fail(SCARL_E_INVALID, "null argument");
+    const int32_t m[3] = {Np[0] / 2, Np[1] / 2, Np[2] / 2};
+    return gen_r_host(h_out, Np, m, xRange, corrL, corrMod, seed, rank, h_phi, flags);
+}
+
+int scarl_hip_gen_std_gauss_fft_mpi(double* h_out, const double xRange[3], const double corrL[3], int32_t corrMod,
+                                    int32_t seed, int32_t rank, const int32_t L[3], const int32_t Np[3],
+                                    int32_t comm_size, const double* h_phi, int32_t flags) {
+    if (!Np || !xRange || !corrL || !L) return fail(SCARL_E_INVALID, "null argument");
+    if (comm_size != 1)
+        return fail(SCARL_E_UNSUPPORTED, "one GPU holds the whole field: comm_size must be 1 (got %d)", comm_size);
+    if (Np[0] != L[0] || Np[1] != L[1] || Np[2] != L[2]) return fail(SCARL_E_INVALID, "Error in Np definition inside FFT_MPI");
+    const int32_t m[3] = {L[0] / 2, L[1] / 2, L[2]};
+    return gen_r_host(h_out, L, m, xRange, corrL, corrMod, seed, rank, h_phi, flags);
+}
+
+int scarl_hip_gen_std_gauss_fft_dev(double* d_out, const int32_t Np[3], const double xRange[3],
+                                    const double corrL[3], int32_t corrMod, int32_t seed, int32_t rank,
+                                    const double* d_phi, int32_t flags) {
+    SCARL_TRY(ensure_init());
+    if (!d_out || !Np || !xRange || !corrL) return fail(SCARL_E_INVALID, "null argument");
+    const int32_t m[3] = {Np[0] / 2, Np[1] / 2, Np[2] / 2};
+    const uint64_t key = (uint64_t)(int64_t)((int64_t)seed + rank);
+    return mode_r_run(d_out, Np, m, xRange, corrL, corrMod, key, d_phi, flags);
+}
+
+// ---- normal_ScaRL ------------------------------------------------------------------------
+int scarl_hip_r8vec_uniform_01(int32_t n, int32_t* seed, double* h_r) {
+    SCARL_TRY(ensure_init());
+    if (!seed || (n > 0 && !h_r)) return fail(SCARL_E_INVALID, "null argument");
+    if (n <= 0) return 0;
+    void *d_r, *d_s;
+    SCARL_TRY(scratch_get("lcg_r", (size_t)n * 8, &d_r));
+    SCARL_TRY(scratch_get("lcg_s", (size_t)n * 4, &d_s));
+    SCARL_TRY(timed("lcg_uniform", [&] {
+        return launch_lcg_uniform(*seed, n, (double*)d_r, (int32_t*)d_s, g_ctx.stream);
+    }));
+    int32_t last = 0;
+    HIP_TRY(hipMemcpyAsync(h_r, d_r, (size_t)n * 8, hipMemcpyDeviceToHost, g_ctx.stream));
+    HIP_TRY(hipMemcpyAsync(&last, (int32_t*)d_s + (n - 1), 4, hipMemcpyDeviceToHost, g_ctx.stream));
+    HIP_TRY(hipStreamSynchronize(g_ctx.stream));
+    *seed = last;
+    return 0;
+}
+
+int scarl_hip_r8_uniform_01(int32_t* seed, double* r) { return scarl_hip_r8vec_uniform_01(1, seed, r); }
+
+int scarl_hip_r8vec_normal_01(int32_t n, int32_t* seed, double* h_x) {
+    // Box-Muller on the reference's pair layout (SRC_F/normal_ScaRL.f90:1186-1254)
+    if (!seed || (n > 0 && !h_x)) return fail(SCARL_E_INVALID, "null argument");
+    if (n <= 0) return 0;
+    const double r8_pi = 3.141592653589793;
+    if (n == 1) {
+        double r[2];
+        SCARL_TRY(scarl_hip_r8vec_uniform_01(2, seed, r));
+        if (r[0] == 0.0) return fail(SCARL_E_INVALID, "R8VEC_NORMAL_01 - Fatal error: R8_UNIFORM_01 returned a value of 0.");
+        h_x[0] = sqrt(-2.0 * log(r[0])) * cos(2.0 * r8_pi * r[1]);
+        return 0;
+    }
+    const int32_t npair = n / 2, m = (n % 2 == 0) ? npair : npair + 1;
+    std::vector<double> r((size_t)2 * m);
+    SCARL_TRY(scarl_hip_r8vec_uniform_01(2 * m, seed, r.data()));
+    for (int32_t p = 0; p < npair; ++p) {
+        const double a = sqrt(-2.0 * log(r[2 * p]));
+        h_x[2 * p] = a * cos(2.0 * r8_pi * r[2 * p + 1]);
+        h_x[2 * p + 1] = a * sin(2.0 * r8_pi * r[2 * p + 1]);
+    }
+    if (n % 2 == 1) h_x[n - 1] = sqrt(-2.0 * log(r[2 * m - 2])) * cos(2.0 * r8_pi * r[2 * m - 1]);
+    return 0;
+}
+
+// ---- main_ScaRL element routines ---------------------------------------------------------
+int scarl_hip_apply_unity_partition(double* h_field, const int32_t Np[3], const int32_t Np_ovlp[3],
+                                    int32_t partition_type, const int32_t topo_pos[3], const int32_t topo_shape[3]) {
+    SCARL_TRY(ensure_init());
+    if (!h_field || !Np || !Np_ovlp || !topo_pos || !topo_shape) return fail(SCARL_E_INVALID, "null argument");
+    if (partition_type != 1) return fail(SCARL_E_UNSUPPORTED, "partitionType %d (only 1, cosine, exists)", partition_type);
+    int lo[3], hi[3];
+    for (int a = 0; a < 3; ++a) {
+        if (Np[a] < 1) return fail(SCARL_E_INVALID, "Np(%d) must be >= 1", a + 1);
+        if (Np_ovlp[a] > Np[a]) return fail(SCARL_E_INVALID, "Np_ovlp(%d) > Np(%d)", a + 1, a + 1);
+        // a neighbour exists on a side iff topo_pos -+ 1 stays inside the topology (:1257-1260)
+        lo[a] = (topo_pos[a] - 1 >= 0 && Np_ovlp[a] >= 2) ? Np_ovlp[a] : 0;
+        hi[a] = (topo_pos[a] + 1 <= topo_shape[a] - 1 && Np_ovlp[a] >= 2) ? Np_ovlp[a] : 0;
+    }
+    const int64_t N = (int64_t)Np[0] * Np[1] * Np[2];
+    void* d;
+    SCARL_TRY(scratch_get("pou", (size_t)N * 8, &d));
+    HIP_TRY(hipMemcpyAsync(d, h_field, (size_t)N * 8, hipMemcpyHostToDevice, g_ctx.stream));
+    SCARL_TRY(timed("pou", [&] {
+        return launch_pou((double*)d, Np[0], Np[1], Np[2], make_int3(lo[0], lo[1], lo[2]),
+                          make_int3(hi[0], hi[1], hi[2]), g_ctx.stream);
+    }));
+    HIP_TRY(hipMemcpyAsync(h_field, d, (size_t)N * 8, hipMemcpyDeviceToHost, g_ctx.stream));
+    HIP_TRY(hipStreamSynchronize(g_ctx.stream));
+    return 0;
+}
+
+int scarl_hip_multivariate_transformation(double avg, double std_dev, int32_t margiFirst, double* h_field, int64_t n) {
+    SCARL_TRY(ensure_init());
+    double mu, sd;
+    if (margiFirst != FOM_GAUSSIAN && margiFirst != FOM_LOGNORMAL)
+        return fail(SCARL_E_INVALID, "First-order marginal not detected (margiFirst = %d)", margiFirst);
+    SCARL_TRY(marginal_params(margiFirst, avg, std_dev, &mu, &sd));
+    if (n <= 0) return 0;
+    if (!h_field) return fail(SCARL_E_INVALID, "null field");
+    void* d;
+    SCARL_TRY(scratch_get("marg", (size_t)n * 8, &d));
+    HIP_TRY(hipMemcpyAsync(d, h_field, (size_t)n * 8, hipMemcpyHostToDevice, g_ctx.stream));
+    SCARL_TRY(timed("marginal", [&] { return launch_marginal((double*)d, n, margiFirst, mu, sd, g_ctx.stream); }));
+    HIP_TRY(hipMemcpyAsync(h_field, d, (size_t)n * 8, hipMemcpyDeviceToHost, g_ctx.stream));
+    HIP_TRY(hipStreamSynchronize(g_ctx.stream));
+    return 0;
+}
+
+// ---- Stat moments ------------------------------------------------------------------------
+int scarl_hip_moments(const double* h_rf, int64_t n, int32_t nmc, double* evnt_avg, double* evnt_std, double* g_avg,
+                      double* g_std) {
+    SCARL_TRY(ensure_init());
+    if (n <= 0 || nmc <= 0) return fail(SCARL_E_INVALID, "n and nmc must be positive");
+    if (!h_rf || !evnt_avg || !evnt_std || !g_avg || !g_std) return fail(SCARL_E_INVALID, "null argument");
+    const int nblocks = (int)std::min<int64_t>(std::max<int64_t>(1, n / (kBlock * 16)), 1024);
+    void *d, *dp;
+    SCARL_TRY(scratch_get("mom_in", (size_t)(n * nmc) * 8, &d));
+    SCARL_TRY(scratch_get("mom_part", (size_t)nblocks * nmc * sizeof(MomPartial), &dp));
+    HIP_TRY(hipMemcpyAsync(d, h_rf, (size_t)(n * nmc) * 8, hipMemcpyHostToDevice, g_ctx.stream));
+    SCARL_TRY(timed("sums", [&] { return launch_sums((const double*)d, n, nmc, nblocks, (MomPartial*)dp, g_ctx.stream); }));
+    std::vector<MomPartial> h((size_t)nblocks * nmc);
+    HIP_TRY(hipMemcpyAsync(h.data(), dp, h.size() * sizeof(MomPartial), hipMemcpyDeviceToHost, g_ctx.stream));
+    HIP_TRY(hipStreamSynchronize(g_ctx.stream));
+    auto nadd_h = [](double& s, double& c, double v) {
+        const double t = s + v;
+        c += (fabs(s) >= fabs(v)) ? (s - t) + v : (v - t) + s;
+        s = t;
+    };
+    double ts = 0, tc = 0, ts2 = 0, tc2 = 0;
+    for (int e = 0; e < nmc; ++e) {
+        double s = 0, c = 0, s2 = 0, c2 = 0;
+        for (int b = 0; b < nblocks; ++b) {
+            const MomPartial& m = h[(size_t)e * nblocks + b];
+            nadd_h(s, c, m.s);
+            c += m.c;
+            nadd_h(s2, c2, m.s2);
+            c2 += m.c2;
+        }
+        const double S = s + c, S2 = s2 + c2;
+        // SRC_F/Stat_ScaRL/statistics_RF.f90:80-87
+        evnt_avg[e] = S / (double)n;
+        evnt_std[e] = sqrt(S2 / (double)n - evnt_avg[e] * evnt_avg[e]);
+        nadd_h(ts, tc, S);
+        nadd_h(ts2, tc2, S2);
+    }
+    const double tot = (double)n * nmc;
+    *g_avg = (ts + tc) / tot;
+    *g_std = sqrt((ts2 + tc2) / tot - (*g_avg) * (*g_avg));
+    return 0;
+}
+
+// ---- special functions -------------------------------------------------------------------
+static int run_fp64_map(int which, int32_t model, double nu, const double* h_x, int32_t n, double* h_out) {
+    SCARL_TRY(ensure_init());
+    if (n <= 0) return 0;
+    if (!h_x || !h_out) return fail(SCARL_E_INVALID, "null argument");
+    void *dx, *dy;
+    SCARL_TRY(scratch_get("map_x", (size_t)n * 8, &dx));
+    SCARL_TRY(scratch_get("map_y", (size_t)n * 8, &dy));
+    HIP_TRY(hipMemcpyAsync(dx, h_x, (size_t)n * 8, hipMemcpyHostToDevice, g_ctx.stream));
+    if (which == 0)
+        SCARL_TRY(timed("kv", [&] { return launch_kv(nu, (const double*)dx, n, (double*)dy, g_ctx.stream); }));
+    else
+        SCARL_TRY(timed("target_corr", [&] {
+            return launch_target_corr(model, nu, (const double*)dx, n, (double*)dy, g_ctx.stream);
+        }));
+    HIP_TRY(hipMemcpyAsync(h_out, dy, (size_t)n * 8, hipMemcpyDeviceToHost, g_ctx.stream));
+    HIP_TRY(hipStreamSynchronize(g_ctx.stream));
+    return 0;
+}
+
+int scarl_hip_kv(double nu, const double* h_x, int32_t n, double* h_out) {
+    if (!(nu >= 0.0)) return fail(SCARL_E_INVALID, "nu must be >= 0");
+    return run_fp64_map(0, 0, nu, h_x, n, h_out);
+}
+
+int scarl_hip_target_corr(int32_t model, double nu, const double* h_rho, int32_t n, double* h_out) {
+    if (model != CM_GAUSSIAN && model != CM_VON_KARMAN && model != CM_EXPONENTIAL)
+        return fail(SCARL_E_INVALID, "unknown correlation model %d", model);
+    if (model == CM_VON_KARMAN && !(nu > 0.0)) return fail(SCARL_E_INVALID, "von Karman needs nu > 0");
+    return run_fp64_map(1, model, nu, h_rho, n, h_out);
+}
+
+// ---- batches -----------------------------------------------------------------------------
+int scarl_hip_batch_create(const scarl_field_desc* fields, int32_t count, int32_t precision, scarl_batch** h) {
+    SCARL_TRY(ensure_init());
+    if (!fields || !h || count <= 0) return fail(SCARL_E_INVALID, "need at least one field descriptor");
+    if (precision != SCARL_PREC_FP32 && precision != SCARL_PREC_FP64)
+        return fail(SCARL_E_INVALID, "precision must be 32 or 64");
+    *h = nullptr;
+    std::unique_ptr<scarl_batch> b(new scarl_batch());
+    b->fp64 = precision == SCARL_PREC_FP64;
+    b->ps.fp64 = b->fp64;
+    b->fields.resize(count);
+    std::vector<double> slot_count(count);
+    for (int i = 0; i < count; ++i) {
+        SCARL_TRY(validate_desc(fields[i]));
+        b->fields[i].d = fields[i];
+        SCARL_TRY(build_field(b.get(), b->fields[i]));
+        b->fields[i].f.mom_slot = i;
+        b->fields[i].f.color = 0;
+        b->fields[i].f.blend = 0;
+        slot_count[i] = (double)fields[i].n[0] * fields[i].n[1] * fields[i].n[2];
+    }
+    SCARL_TRY(finish_amplitudes(b.get()));
+    SCARL_TRY(finalize_batch(b.get(), slot_count));
+    *h = b.release();
+    return 0;
+}
+
+int scarl_hip_blend_create(const int32_t gdim[3], const int32_t nb[3], const int32_t* starts, const int32_t* exts,
+                           const scarl_field_desc* params, int32_t precision, scarl_batch** h) {
+    SCARL_TRY(ensure_init());
+    if (!gdim || !nb || !starts || !exts || !params || !h) return fail(SCARL_E_INVALID, "null argument");
+    if (precision != SCARL_PREC_FP32 && precision != SCARL_PREC_FP64)
+        return fail(SCARL_E_INVALID, "precision must be 32 or 64");
+    *h = nullptr;
+    // validate the tensor-product tiling: positions cover [0, gdim) with overlaps only between
+    // neighbours; overlaps of 1 point are rejected (no ramp exists, SRC_F/main_ScaRL.f90:1255)
+    std::vector<int> st[3], ex[3], lo[3], hi[3];
+    int off = 0;
+    for (int a = 0; a < 3; ++a) {
+        if (nb[a] < 1 || gdim[a] < 1) return fail(SCARL_E_INVALID, "bad grid on axis %d", a);
+        for (int p = 0; p < nb[a]; ++p) {
+            st[a].push_back(starts[off + p]);
+            ex[a].push_back(exts[off + p]);
+        }
+        off += nb[a];
+        lo[a].assign(nb[a], 0);
+        hi[a].assign(nb[a], 0);
+        if (st[a][0] != 0) return fail(SCARL_E_INVALID, "axis %d: first subdomain must start at 0", a);
+        if (st[a].back() + ex[a].back() != gdim[a])
+            return fail(SCARL_E_INVALID, "axis %d: last subdomain must end at %d", a, gdim[a]);
+        for (int p = 0; p < nb[a]; ++p) {
+            if (ex[a][p] < 1) return fail(SCARL_E_INVALID, "axis %d: empty subdomain", a);
+            if (p > 0) {
+                const int ov = st[a][p - 1] + ex[a][p - 1] - st[a][p];
+                if (ov < 0) return fail(SCARL_E_INVALID, "axis %d: gap between subdomains %d and %d", a, p - 1, p);
+                if (ov == 1) return fail(SCARL_E_UNSUPPORTED, "axis %d: overlap of one point has no PoU ramp", a);
+                if (st[a][p] <= st[a][p - 1]) return fail(SCARL_E_INVALID, "axis %d: starts must increase", a);
+                if (p > 1 && st[a][p] < st[a][p - 2] + ex[a][p - 2])
+                    return fail(SCARL_E_INVALID, "axis %d: overlap reaches beyond the neighbour", a);
+                if (2 * ov > ex[a][p] || 2 * ov > ex[a][p - 1])
+                    return fail(SCARL_E_INVALID, "axis %d: overlap %d larger than half a subdomain", a, ov);
+                lo[a][p] = ov;
+                hi[a][p - 1] = ov;
+            }
+        }
+    }
+    std::unique_ptr<scarl_batch> b(new scarl_batch());
+    b->fp64 = precision == SCARL_PREC_FP64;
+    b->ps.fp64 = b->fp64;
+    b->global = true;
+    for (int a = 0; a < 3; ++a) b->gdim[a] = gdim[a];
+    const int count = nb[0] * nb[1] * nb[2];
+    b->fields.resize(count);
+    int id = 0;
+    for (int k = 0; k < nb[2]; ++k)
+        for (int j = 0; j < nb[1]; ++j)
+            for (int i = 0; i < nb[0]; ++i, ++id) {
+                scarl_field_desc d = *params;
+                d.n[0] = ex[0][i];
+                d.n[1] = ex[1][j];
+                d.n[2] = ex[2][k];
+                d.stream = (uint32_t)id;
+                SCARL_TRY(validate_desc(d));
+                FieldHost& fh = b->fields[id];
+                fh.d = d;
+                SCARL_TRY(build_field(b.get(), fh));
+                FieldH& f = fh.f;
+                f.blend = 1;
+                f.color = (i & 1) | ((j & 1) << 1) | ((k & 1) << 2);
+                const int pos[3] = {i, j, k};
+                for (int a = 0; a < 3; ++a) {
+                    f.nov_lo[a] = lo[a][pos[a]];
+                    f.nov_hi[a] = hi[a][pos[a]];
+                }
+                f.mom_slot = 0;
+            }
+    SCARL_TRY(finish_amplitudes(b.get()));
+    std::vector<double> slot_count(1, (double)gdim[0] * gdim[1] * gdim[2]);
+    SCARL_TRY(finalize_batch(b.get(), slot_count));
+    // output placement inside the global field
+    const size_t rsz = b->fp64 ? 8 : 4;
+    id = 0;
+    for (int k = 0; k < nb[2]; ++k)
+        for (int j = 0; j < nb[1]; ++j)
+            for (int i = 0; i < nb[0]; ++i, ++id) {
+                FieldH& f = b->fields[id].f;
+                const int64_t o = st[0][i] + (int64_t)gdim[0] * (st[1][j] + (int64_t)gdim[1] * st[2][k]);
+                f.out = static_cast<char*>(b->d_out) + o * rsz;
+                f.out_ld1 = gdim[0];
+                f.out_ld2 = (int64_t)gdim[0] * gdim[1];
+            }
+    SCARL_TRY(upload_fields(b.get()));
+    *h = b.release();
+    return 0;
+}
+
+int scarl_hip_batch_set_noise(scarl_batch* b, int32_t field, const void* h_noise) {
+    SCARL_TRY(ensure_init());
+    if (!b || field < 0 || field >= (int)b->fields.size()) return fail(SCARL_E_INVALID, "bad batch or field index");
+    FieldHost& fh = b->fields[field];
+    const size_t bytes = (size_t)fh.f.h1 * fh.f.n[1] * fh.f.n[2] * 2 * (b->fp64 ? 8 : 4);
+    if (h_noise) {
+        if (!fh.noise_buf) SCARL_TRY(dmalloc(&fh.noise_buf, bytes));
+        HIP_TRY(hipMemcpyAsync(fh.noise_buf, h_noise, bytes, hipMemcpyHostToDevice, g_ctx.stream));
+        fh.f.W = fh.noise_buf;
+        fh.f.src_mode = SRC_HOST_NOISE;
+    } else {
+        fh.f.W = nullptr;
+        fh.f.src_mode = SRC_DEVICE_NOISE;
+    }
+    HIP_TRY(hipStreamSynchronize(g_ctx.stream));
+    SCARL_TRY(upload_fields(b));
+    return 0;
+}
+
+int scarl_hip_batch_run(scarl_batch* b, int32_t use_graph) {
+    SCARL_TRY(ensure_init());
+    if (!b) return fail(SCARL_E_INVALID, "null batch");
+    if (!use_graph || g_ctx.timing) return enqueue_batch(b);
+    if (!b->gexec) {
+        HIP_TRY(hipStreamBeginCapture(g_ctx.stream, hipStreamCaptureModeThreadLocal));
+        int rc = enqueue_batch(b);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(g_ctx.stream, &g);
+        if (rc) {
+            if (g) hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return fail(SCARL_E_HIP, "graph capture: %s", hipGetErrorString(e));
+        b->graph = g;
+        e = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            drop_graph(b);
+            return fail(SCARL_E_HIP, "graph instantiate: %s", hipGetErrorString(e));
+        }
+    }
+    HIP_TRY(hipGraphLaunch(b->gexec, g_ctx.stream));
+    return 0;
+}
+
+int scarl_hip_batch_sync(scarl_batch* b) {
+    SCARL_TRY(ensure_init());
+    (void)b;
+    HIP_TRY(hipStreamSynchronize(g_ctx.stream));
+    return 0;
+}
+
+int scarl_hip_batch_output(scarl_batch* b, int32_t field, void** d_ptr, int64_t dims[3]) {
+    if (!b || !d_ptr || !dims) return fail(SCARL_E_INVALID, "null argument");
+    if (field < 0) {
+        if (!b->global) return fail(SCARL_E_INVALID, "batch has no global field");
+        *d_ptr = b->d_out;
+        for (int a = 0; a < 3; ++a) dims[a] = b->gdim[a];
+        return 0;
+    }
+    if (field >= (int)b->fields.size()) return fail(SCARL_E_INVALID, "field index out of range");
+    if (b->global) return fail(SCARL_E_INVALID, "blended subdomains are only available as the global field");
+    *d_ptr = b->fields[field].f.out;
+    for (int a = 0; a < 3; ++a) dims[a] = b->fields[field].f.n[a];
+    return 0;
+}
+
+int scarl_hip_batch_copy_output(scarl_batch* b, int32_t field, void* h_dst, int64_t nbytes) {
+    SCARL_TRY(ensure_init());
+    void* p;
+    int64_t dims[3];
+    SCARL_TRY(scarl_hip_batch_output(b, field, &p, dims));
+    const int64_t need = dims[0] * dims[1] * dims[2] * (b->fp64 ? 8 : 4);
+    if (!h_dst || nbytes < need) return fail(SCARL_E_INVALID, "destination holds %lld bytes, need %lld",
+                                             (long long)nbytes, (long long)need);
+    HIP_TRY(hipMemcpyAsync(h_dst, p, (size_t)need, hipMemcpyDeviceToHost, g_ctx.stream));
+    HIP_TRY(hipStreamSynchronize(g_ctx.stream));
+    return 0;
+}
+
+int scarl_hip_batch_moments(scarl_batch* b, double* h_out, int32_t max_slots, int32_t* count) {
+    SCARL_TRY(ensure_init());
+    if (!b || !h_out || !count) return fail(SCARL_E_INVALID, "null argument");
+    const int n = std::min(max_slots, b->nslots);
+    HIP_TRY(hipMemcpyAsync(h_out, b->d_slot_out, (size_t)n * 5 * sizeof(double), hipMemcpyDeviceToHost,
+                           g_ctx.stream));
+    HIP_TRY(hipStreamSynchronize(g_ctx.stream));
+    *count = b->nslots;
+    return 0;
+}
+
+int scarl_hip_batch_field_info(scarl_batch* b, int32_t field, double* amp0, double* spectrum_sum) {
+    if (!b || field < 0 || field >= (int)b->fields.size()) return fail(SCARL_E_INVALID, "bad batch or field index");
+    if (amp0) *amp0 = b->fields[field].f.amp0;
+    if (spectrum_sum) *spectrum_sum = b->fields[field].spec_sum;
+    return 0;
+}
+
+int scarl_hip_batch_bytes(scarl_batch* b, int64_t* bytes) {
+    if (!b || !bytes) return fail(SCARL_E_INVALID, "null argument");
+    *bytes = b->bytes;
+    return 0;
+}
+
+int scarl_hip_batch_destroy(scarl_batch* b) {
+    if (!b) return 0;
+    if (g_ctx.ready) hipStreamSynchronize(g_ctx.stream);
+    delete b;
+    return 0;
+}
+
+int scarl_hip_gen_field_h(void* h_out, const scarl_field_desc* d, int32_t precision, const void* h_noise) {
+    if (!h_out || !d) return fail(SCARL_E_INVALID, "null argument");
+    scarl_batch* b = nullptr;
+    SCARL_TRY(scarl_hip_batch_create(d, 1, precision, &b));
+    std::unique_ptr<scarl_batch> guard(b);
+    if (h_noise) SCARL_TRY(scarl_hip_batch_set_noise(b, 0, h_noise));
+    SCARL_TRY(scarl_hip_batch_run(b, 0));
+    const int64_t need = (int64_t)d->n[0] * d->n[1] * d->n[2] * (precision == SCARL_PREC_FP64 ? 8 : 4);
+    SCARL_TRY(scarl_hip_batch_copy_output(b, 0, h_out, need));
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,305 @@
+"""GPU parity: the HIP path through the C ABI against the FP64 oracle on the same inputs.
+
+Bars: FP64 paths <= 1e-12 relative L2 (reference-parity REDFT01 and Hermitian C2R);
+integer LCG bit-exact; FP32 Hermitian path <= 1e-5 relative L2 against the FP64 oracle on the
+same white noise.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+FP32_TOL = 1e-5
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+# ---- fftw_ScaRL: gen_std_gauss_FFT (REDFT01, FP64) --------------------------------------
+@pytest.mark.parametrize("Np", [(24, 20, 16), (5, 8, 6), (41, 41, 41), (81, 30, 17), (2, 3, 4), (1, 6, 6),
+                                (64, 64, 64), (121, 40, 9)])
+def test_gen_std_gauss_fft_host_phi(hip, Np):
+    from scarl_amd import fftw_ScaRL
+    _say(f"gen_std_gauss_FFT Np={Np}")
+    rng = np.random.default_rng(sum(Np))
+    phi = np.asfortranarray(rng.random(tuple(v // 2 for v in Np)))
+    xr = tuple(float(v - 1) for v in Np)
+    cl = (3.0, 2.5, 1.5)
+    gpu = np.empty(Np, np.float64, order="F")
+    fftw_ScaRL.gen_std_gauss_FFT(gpu, Np, xr, cl, 1, 1, 0, phi=phi)
+    ref = O.gen_std_gauss_fft(Np, xr, cl, 1, phi)
+    if np.linalg.norm(ref) == 0:
+        assert np.all(gpu == 0)
+    else:
+        assert rel_l2(gpu, ref) <= FP64_TOL
+
+
+def test_gen_std_gauss_fft_256_cube(hip):
+    """256^3 anisotropic Gaussian (corrL 16,16,8 cells), seed-7 phases, lognormal marginal."""
+    from scarl_amd import fftw_ScaRL
+    _say("gen_std_gauss_FFT 256^3")
+    Np = (256, 256, 256)
+    phi = np.asfortranarray(np.random.default_rng(7).random((128, 128, 128)))
+    xr, cl = (255.0, 255.0, 255.0), (16.0, 16.0, 8.0)
+    gpu = np.empty(Np, np.float64, order="F")
+    fftw_ScaRL.gen_std_gauss_FFT(gpu, Np, xr, cl, 1, 7, 0, phi=phi)
+    ref = O.gen_std_gauss_fft(Np, xr, cl, 1, phi)
+    assert rel_l2(gpu, ref) <= FP64_TOL
+    # the marginal fused into the z pass equals the separate transform
+    fftw_ScaRL.set_marginal(2, 1.0, 0.3)
+    fused = np.empty(Np, np.float64, order="F")
+    fftw_ScaRL.gen_std_gauss_FFT(fused, Np, xr, cl, 1, 7, 0, phi=phi, fuse_marginal=True)
+    assert rel_l2(fused, O.marginal(ref, 2, 1.0, 0.3)) <= FP64_TOL
+
+
+def test_gen_std_gauss_fft_device_phi(hip):
+    from scarl_amd import fftw_ScaRL
+    Np = (20, 18, 12)
+    gpu = np.empty(Np, np.float64, order="F")
+    fftw_ScaRL.gen_std_gauss_FFT(gpu, Np, (19.0, 17.0, 11.0), (2.0, 2.0, 2.0), 1, 5, 3)
+    phi = O.phi_octant((10, 9, 6), 5 + 3, 0)
+    ref = O.gen_std_gauss_fft(Np, (19.0, 17.0, 11.0), (2.0, 2.0, 2.0), 1, phi)
+    assert rel_l2(gpu, ref) <= FP64_TOL
+
+
+def test_gen_std_gauss_fft_mpi_one_rank(hip):
+    from scarl_amd import fftw_ScaRL
+    L = (16, 12, 10)
+    phi = np.asfortranarray(np.random.default_rng(3).random((8, 6, 10)))
+    gpu = fftw_ScaRL.gen_Std_Gauss_FFT_MPI((15.0, 11.0, 9.0), (1.0, 1.0, 1.0), 1, 1, 0, L, L, 1, phi=phi)
+    ref = O.gen_std_gauss_fft_mpi(L, (15.0, 11.0, 9.0), (1.0, 1.0, 1.0), 1, phi)
+    assert rel_l2(gpu, ref) <= FP64_TOL
+
+
+def test_gen_std_gauss_fft_errors(hip):
+    from scarl_amd import fftw_ScaRL, ScarlError
+    out = np.empty((8, 8, 8), np.float64, order="F")
+    with pytest.raises(ScarlError):
+        fftw_ScaRL.gen_std_gauss_FFT(out, (8, 8, 8), (7.0, 7.0, 7.0), (1.0, 1.0, 1.0), 2, 1, 0)  # von Karman
+    with pytest.raises(ScarlError):
+        fftw_ScaRL.gen_Std_Gauss_FFT_MPI((7.0, 7.0, 7.0), (1.0, 1.0, 1.0), 1, 1, 0, (8, 8, 8), (8, 8, 8), 2)
+
+
+# ---- normal_ScaRL ------------------------------------------------------------------------
+@pytest.mark.parametrize("n,seed", [(1, 12345), (3, 12345), (2, 1), (1000, 777), (100_000, -123456789),
+                                    (5, 2147483646)])
+def test_lcg_bit_exact(hip, n, seed):
+    from scarl_amd import normal_ScaRL
+    r, s = normal_ScaRL.r8vec_uniform_01(n, seed)
+    ro, so = O.r8vec_uniform_01(n, seed)
+    assert s == so
+    np.testing.assert_array_equal(r, ro)
+    x, s = normal_ScaRL.r8vec_normal_01(n, seed)
+    xo, so = O.r8vec_normal_01(n, seed)
+    assert s == so
+    np.testing.assert_allclose(x, xo, rtol=1e-15, atol=1e-15)
+
+
+def test_lcg_known_answer(hip):
+    from scarl_amd import normal_ScaRL
+    r, s = normal_ScaRL.r8_uniform_01(12345)
+    assert s == 207482415
+    r, s = normal_ScaRL.r8vec_uniform_01(3, 12345)
+    assert s == 2035175616
+
+
+# ---- main_ScaRL element routines ---------------------------------------------------------
+@pytest.mark.parametrize("shape", [(2, 1, 1), (2, 2, 1), (2, 2, 2), (3, 3, 3)])
+def test_unity_partition(hip, shape):
+    from scarl_amd import main_ScaRL
+    Np, Nov = (20, 17, 12), (5, 4, 3)
+    rng = np.random.default_rng(1)
+    for pos in [tuple(p) for p in np.ndindex(*shape)][:5]:
+        f = np.asfortranarray(rng.standard_normal(Np))
+        ref = O.apply_pou(f, Nov, pos, shape)
+        main_ScaRL.apply_UnityPartition_mtx(Np, Nov, 1, pos, shape, f)
+        np.testing.assert_allclose(f, ref, rtol=1e-15, atol=1e-15)
+
+
+def test_marginal(hip):
+    from scarl_amd import main_ScaRL, ScarlError
+    x = np.asfortranarray(np.random.default_rng(2).standard_normal((30, 20, 10)))
+    for margi, avg, sd in ((1, 2.0, 0.5), (2, 1.0, 0.3), (2, 3.0, 1.5)):
+        y = main_ScaRL.multiVariateTransformation(avg, sd, margi, x.copy(order="F"))
+        np.testing.assert_allclose(y, O.marginal(x, margi, avg, sd), rtol=1e-15)
+    with pytest.raises(ScarlError):
+        main_ScaRL.multiVariateTransformation(-1.0, 0.3, 2, x.copy(order="F"))
+
+
+def test_moments_compensated(hip):
+    from scarl_amd import statistics
+    rng = np.random.default_rng(4)
+    rf = np.asfortranarray(1e8 + rng.standard_normal((1_000_003, 3)) * 1e-3)
+    ea, es, ga, gs = statistics.calculate_average_and_stdVar(rf)
+    oa, os_, oga, ogs = O.moments(rf)
+    np.testing.assert_allclose(ea, oa, rtol=1e-15)
+    np.testing.assert_allclose(ga, oga, rtol=1e-15)
+    # the reference's E[x^2]-E[x]^2 cancels: both sides see the same cancellation
+    np.testing.assert_allclose(es, os_, rtol=1e-3, atol=1e-3)
+
+
+# ---- special functions -------------------------------------------------------------------
+def test_kv_device(hip):
+    from scarl_amd import statistics
+    xs = np.array([1e-3, 0.05, 0.5, 1.0, 1.9, 3.0, 6.0, 8.9, 9.1, 20.0, 40.0, 80.0])
+    for nu in (0.0, 0.1, 0.3, 0.5, 1.0, 1.5, 2.7):
+        dev = statistics.kv(nu, xs)
+        for x, d in zip(xs, dev):
+            ref = O.kv(nu, x)
+            tol = 1e-7 if 2 < x <= 9 else (2e-9 if 9 < x < 20 else 1e-13)
+            assert abs(d - ref) <= tol * abs(ref), (nu, x, d, ref)
+
+
+def test_target_corr_device(hip):
+    from scarl_amd import statistics
+    rho = np.linspace(0, 4, 41)
+    for model, nu in ((1, 0.5), (3, 0.5), (2, 0.3), (2, 1.5)):
+        dev = statistics.target_corr(model, rho, nu)
+        ref = [O.target_corr(model, 1, (r, 0, 0), (1, 1, 1), nu) for r in rho]
+        np.testing.assert_allclose(dev, ref, rtol=1e-7, atol=1e-12)
+
+
+# ---- Hermitian generator -----------------------------------------------------------------
+HSHAPES = [(64, 48, 1), (256, 256, 1), (16, 12, 10), (32, 24, 20), (16, 9, 7), (40, 44, 36), (64, 64, 64)]
+
+
+@pytest.mark.parametrize("n", HSHAPES)
+@pytest.mark.parametrize("model,nu", [(1, 0.5), (2, 0.3), (3, 0.5)])
+def test_hermitian_fp64_host_noise(hip, n, model, nu):
+    from scarl_amd import generator
+    _say(f"hermitian fp64 n={n} model={model}")
+    L = (8.0, 6.0, 4.0)
+    spec = generator.FieldSpec(n=n, model=model, corrL=L, nu=nu, seed=11)
+    W = O.noise_hermitian(n, 11, 0)
+    gpu = generator.gen_field_h(spec, 64, W)
+    ref = O.gen_mode_h(n, (1, 1, 1), model, L, nu, W=W)
+    assert rel_l2(gpu, ref) <= FP64_TOL
+
+
+@pytest.mark.parametrize("n", [(32, 24, 20), (256, 256, 1), (48, 40, 30)])
+def test_hermitian_fp64_device_noise(hip, n):
+    from scarl_amd import generator
+    spec = generator.FieldSpec(n=n, model=2, corrL=(6.0, 5.0, 4.0), nu=0.3, seed=3, stream=9, margi=2, avg=1.0,
+                               std_dev=0.3)
+    gpu = generator.gen_field_h(spec, 64)
+    ref = O.gen_mode_h(n, (1, 1, 1), 2, (6.0, 5.0, 4.0), 0.3, seed=3, stream=9, margi=2, avg=1.0, std=0.3)
+    assert rel_l2(gpu, ref) <= FP64_TOL
+
+
+@pytest.mark.parametrize("n,model", [((64, 64, 64), 2), ((128, 96, 80), 3), ((256, 256, 1), 1), ((176, 144, 160), 3)])
+def test_hermitian_fp32_vs_fp64_oracle(hip, n, model):
+    from scarl_amd import generator
+    _say(f"hermitian fp32 n={n}")
+    L = (12.0, 10.0, 8.0)
+    spec = generator.FieldSpec(n=n, model=model, corrL=L, nu=0.3, seed=5, margi=2, avg=1.0, std_dev=0.5)
+    W = O.noise_hermitian(n, 5, 0)
+    gpu = generator.gen_field_h(spec, 32, W.astype(np.complex64))
+    ref = O.gen_mode_h(n, (1, 1, 1), model, L, 0.3, W=W.astype(np.complex64).astype(np.complex128), margi=2,
+                       avg=1.0, std=0.5)
+    err = rel_l2(gpu, ref)
+    _say(f"  fp32 rel L2 = {err:.3e}, max abs = {np.abs(gpu - ref).max():.3e}")
+    assert err <= FP32_TOL
+
+
+def test_hermitian_fp32_device_noise(hip):
+    from scarl_amd import generator
+    n = (96, 64, 48)
+    spec = generator.FieldSpec(n=n, model=2, corrL=(16.0, 16.0, 16.0), nu=0.3, seed=77)
+    gpu = generator.gen_field_h(spec, 32)
+    ref = O.gen_mode_h(n, (1, 1, 1), 2, (16.0, 16.0, 16.0), 0.3, seed=77)
+    assert rel_l2(gpu, ref) <= FP32_TOL
+
+
+# ---- batches -----------------------------------------------------------------------------
+def test_ensemble_batch_mixed_sizes(hip):
+    from scarl_amd import generator
+    specs = []
+    sizes = [64, 96, 128, 160, 192, 96]
+    for i, e in enumerate(sizes):
+        model = 1 if i % 2 == 0 else 3
+        L = 12.0 if model == 1 else 20.0
+        specs.append(generator.FieldSpec(n=(e, e, e), model=model, corrL=(L, L, L), seed=1000 + i, stream=i,
+                                         margi=1, avg=0.0, std_dev=1.0))
+    for prec, tol in ((64, FP64_TOL), (32, FP32_TOL)):
+        with generator.Batch.ensemble(specs, prec) as b:
+            b.run()
+            mom = b.moments()
+            for i, s in enumerate(specs):
+                out = b.output(i)
+                ref = O.gen_mode_h(s.shape, (1, 1, 1), s.model, s.corrL, seed=s.seed, stream=s.stream, margi=1,
+                                   avg=0.0, std=1.0)
+                assert rel_l2(out, ref) <= tol, (prec, i)
+                x = out.astype(np.float64)
+                assert abs(mom[i, 3] - x.mean()) <= 1e-12 * max(1.0, abs(x).max())
+                assert abs(mom[i, 2] - x.size) == 0
+
+
+def test_ensemble_graph_replay_matches_eager(hip):
+    from scarl_amd import generator
+    specs = [generator.FieldSpec(n=(64, 48, 40), model=2, corrL=(8.0, 8.0, 8.0), nu=0.3, seed=9, stream=i)
+             for i in range(3)]
+    with generator.Batch.ensemble(specs, 32) as b:
+        b.run(use_graph=False)
+        eager = [b.output(i) for i in range(3)]
+        for rep in range(3):
+            b.run(use_graph=True)
+            b.sync()
+            for i in range(3):
+                np.testing.assert_array_equal(b.output(i), eager[i])
+            b.run(use_graph=False)  # eager work between replays
+            b.sync()
+
+
+def _blend_reference(gdim, starts, exts, spec):
+    fields, st, ws = [], [], []
+    sid = 0
+    for k in range(len(starts[2])):
+        for j in range(len(starts[1])):
+            for i in range(len(starts[0])):
+                n = (exts[0][i], exts[1][j], exts[2][k])
+                f = O.gen_mode_h(n, (1, 1, 1), spec.model, spec.corrL, spec.nu, seed=spec.seed, stream=sid)
+                pos = (i, j, k)
+                w = []
+                for a in range(3):
+                    p = pos[a]
+                    lo = starts[a][p - 1] + exts[a][p - 1] - starts[a][p] if p > 0 else 0
+                    hi = starts[a][p] + exts[a][p] - starts[a][p + 1] if p + 1 < len(starts[a]) else 0
+                    w.append(O.pou_weights_1d(n[a], lo, hi, lo > 0, hi > 0))
+                fields.append(f)
+                st.append((starts[0][i], starts[1][j], starts[2][k]))
+                ws.append(w)
+                sid += 1
+    G = O.blend_global(fields, st, gdim, ws)
+    return O.marginal(G, spec.margi, spec.avg, spec.std_dev) if spec.margi else G
+
+
+@pytest.mark.parametrize("prec,tol", [(64, FP64_TOL), (32, FP32_TOL)])
+def test_blend_subdomains(hip, prec, tol):
+    from scarl_amd import generator
+    starts = [[0, 40, 80], [0, 36], [0, 24]]
+    exts = [[48, 48, 40], [44, 44], [32, 24]]
+    gdim = (120, 80, 48)
+    spec = generator.FieldSpec(n=(2, 2, 2), model=3, corrL=(10.0, 10.0, 10.0), seed=21, margi=2, avg=1.0,
+                               std_dev=0.5)
+    with generator.Batch.blend(gdim, starts, exts, spec, prec) as b:
+        b.run()
+        G = b.output(-1)
+        mom = b.moments()
+    ref = _blend_reference(gdim, starts, exts, spec)
+    assert rel_l2(G, ref) <= tol
+    g = G.astype(np.float64)
+    assert abs(mom[0, 3] - g.mean()) <= 1e-10 * abs(g.mean())
+    assert mom[0, 2] == g.size
+
+
+def test_blend_partition_of_unity_variance(hip):
+    """With the same field on every subdomain the blend must give back sqrt(sum w) * f = f where
+    the weights sum to one: check on a Gaussian field via sum of squared weights == 1."""
+    w_lo = O.pou_weights_1d(48, 8, 8, True, False)[:8]
+    w_hi = O.pou_weights_1d(48, 8, 8, False, True)[-8:]
+    np.testing.assert_allclose(w_lo + w_hi, 1.0, atol=1e-15)

@@ -1,0 +1,367 @@
+"""The FP64 CPU oracle against the reference's golden vectors, the reference routines compiled
+from source (oracle/_ref), library routines and closed forms.  Each check is chosen so that a
+plausible mistake (dropped term, wrong sign/index, transposed operand) fails it."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+G = np.load(os.path.join(os.path.dirname(__file__), "golden", "golden.npz"))
+
+
+# ---- LCG (SRC_F/normal_ScaRL.f90) -------------------------------------------------------
+def test_lcg_known_answer():
+    # normal_ScaRL.f90:762-769
+    r, s = O.r8vec_uniform_01(3, 12345)
+    assert s == 2035175616
+    # the table prints 6 digits (its last digit is not consistently rounded)
+    assert np.all(np.abs(r - G["lcg_r"]) <= 1e-6)
+    seeds = []
+    sd = 12345
+    for _ in range(3):
+        _, sd = O.r8vec_uniform_01(1, sd)
+        seeds.append(sd)
+    assert seeds == list(G["lcg_seeds_out"])
+
+
+def test_lcg_closed_form_jump():
+    # seed_i = seed0 * 16807^i mod (2^31 - 1) for positive and negative seeds
+    for seed0 in (1, 12345, 2147483646, -7, -123456789):
+        r, s = O.r8vec_uniform_01(50, seed0)
+        m = 2147483647
+        expect = [(seed0 % m) * pow(16807, i + 1, m) % m for i in range(50)]
+        assert s == expect[-1]
+        np.testing.assert_array_equal(r, np.array(expect, np.float64) * 4.656612875e-10)
+
+
+@pytest.mark.skipif(O.ref_lib() is None, reason="reference sources not compiled (oracle/_ref)")
+def test_lcg_matches_reference_build():
+    import ctypes as C
+    R = O.ref_lib()
+    for n, seed in ((1, 5), (2, 777), (7, 1), (10, 123456789), (33, 42)):
+        sd = C.c_int(seed)
+        rr = np.empty(n)
+        R.ref_r8vec_uniform_01(n, C.byref(sd), rr.ctypes.data_as(C.c_void_p))
+        r, s = O.r8vec_uniform_01(n, seed)
+        np.testing.assert_array_equal(r, rr)
+        assert s == sd.value
+        sd = C.c_int(seed)
+        xx = np.empty(n)
+        R.ref_r8vec_normal_01(n, C.byref(sd), xx.ctypes.data_as(C.c_void_p))
+        x, s = O.r8vec_normal_01(n, seed)
+        np.testing.assert_array_equal(x, xx)
+        assert s == sd.value
+
+
+def test_box_muller_layout_brute_force():
+    # pairs (cos, sin) of one (r1, r2) pair; odd n consumes one extra pair for the tail
+    for n in (1, 2, 5, 6):
+        x, s = O.r8vec_normal_01(n, 99)
+        npairs = n // 2 if n % 2 == 0 else n // 2 + 1
+        r, s2 = O.r8vec_uniform_01(2 * npairs if n > 1 else 2, 99)
+        assert s == s2
+        exp = []
+        for p in range(n // 2):
+            a = np.sqrt(-2 * np.log(r[2 * p]))
+            exp += [a * np.cos(2 * np.pi * r[2 * p + 1]), a * np.sin(2 * np.pi * r[2 * p + 1])]
+        if n % 2:
+            exp.append(np.sqrt(-2 * np.log(r[-2])) * np.cos(2 * np.pi * r[-1]))
+        np.testing.assert_allclose(x, exp, rtol=1e-15, atol=1e-15)
+
+
+# ---- Philox ------------------------------------------------------------------------------
+def test_philox_random123_kat():
+    assert list(O.philox4x32_10([0, 0, 0, 0], [0, 0])) == [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]
+    assert list(O.philox4x32_10([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2)) == [0x408F276D, 0x41C83B0E, 0xA20BC7C6,
+                                                                         0x6D5451FD]
+    assert list(O.philox4x32_10([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], [0xA4093822, 0x299F31D0])) == [
+        0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]
+
+
+def test_noise_moments_and_hermitian():
+    n = (16, 12, 10)
+    W = O.noise_hermitian(n, 5, 3)
+    # Hermitian on the kx = 0 and kx = n1/2 planes, real at self-conjugate points
+    for kx in (0, 8):
+        P = W[kx]
+        Pm = np.conj(P[(-np.arange(12)) % 12][:, (-np.arange(10)) % 10])
+        np.testing.assert_array_equal(P, Pm)
+    # irfftn of it is real-consistent: full-spectrum Hermitian extension gives a real field
+    f = np.fft.irfftn(np.transpose(W, (2, 1, 0)), s=(10, 12, 16), axes=(0, 1, 2))
+    full = np.fft.fftn(f)
+    np.testing.assert_allclose(full[:, :, :9], np.transpose(W, (2, 1, 0)), atol=1e-12)
+    big = O.noise_hermitian((64, 64, 32), 1, 0)
+    off = big[1:32]
+    assert abs(np.mean(np.abs(off) ** 2) - 1.0) < 0.01
+    assert abs(np.mean(off.real)) < 0.01 and abs(np.mean(off.imag)) < 0.01
+
+
+# ---- FFT / REDFT01 -----------------------------------------------------------------------
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 8, 16, 41, 81, 91, 121, 144, 176])
+def test_redft01_vs_scipy_and_brute_force(n):
+    x = G[f"dct3_in_{n}"]
+    ref = G[f"dct3_out_{n}"]
+    np.testing.assert_allclose(O.redft01_fast(x), ref, rtol=0, atol=1e-12 * max(1, np.abs(ref).max()))
+    np.testing.assert_allclose(O.redft01_direct(x), ref, rtol=0, atol=1e-12 * max(1, np.abs(ref).max()))
+
+
+def test_redft01_definition_small():
+    # closed form Y_k = X_0 + 2 sum_{j>=1} X_j cos(pi j (k+1/2)/n) at n = 3
+    x = np.array([0.3, -1.2, 2.0])
+    k = np.arange(3)
+    exp = x[0] + 2 * (x[1] * np.cos(np.pi * 1 * (k + 0.5) / 3) + x[2] * np.cos(np.pi * 2 * (k + 0.5) / 3))
+    np.testing.assert_allclose(O.redft01_fast(x), exp, atol=1e-15)
+
+
+@pytest.mark.parametrize("n", [2, 3, 4, 12, 41, 64, 81, 96, 160, 176, 192, 256])
+def test_fft_vs_brute_force(n):
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    d = O.dft1d_direct(x, +1)
+    np.testing.assert_allclose(O.fft1d(x, +1), d, atol=1e-13 * n)
+    np.testing.assert_allclose(O.fft1d(x, -1), O.dft1d_direct(x, -1), atol=1e-13 * n)
+
+
+def test_redft01_3d_octant_vs_scipy():
+    rng = np.random.default_rng(3)
+    a = np.asfortranarray(rng.standard_normal((9, 8, 7)))
+    a[4:] = 0
+    a[:, 4:] = 0
+    a[:, :, 3:] = 0
+    import scipy.fft as sf
+    np.testing.assert_allclose(O.redft01_3d(a, (4, 4, 3)), sf.dctn(a, type=3), atol=1e-12)
+
+
+@pytest.mark.parametrize("key", ["8x6x1", "10x8x6", "16x12x9", "12x7x5"])
+def test_c2r_vs_irfftn(key):
+    H = G[f"c2r_in_{key}"]
+    ref = G[f"c2r_out_{key}"]
+    n = tuple(int(v) for v in key.split("x"))
+    out = O.c2r_3d(np.asfortranarray(H), n)
+    np.testing.assert_allclose(out, ref, atol=1e-10 * np.abs(ref).max())
+
+
+# ---- mode R generator --------------------------------------------------------------------
+def test_mode_r_amplitude_literal_points():
+    # S = prod corrL exp(-PI k^2), k = n*dk/corrL, dk = 1/xRange; amp = sqrt(S) cos(2 PI phi)
+    Np = (6, 5, 4)
+    xr = (5.0, 8.0, 3.0)
+    cl = (1.0, 2.0, 0.5)
+    phi = np.asfortranarray(np.random.default_rng(1).random((3, 2, 2)))
+    A = O.mode_r_amplitude(Np, (3, 2, 2), xr, cl, 1, phi)
+    PI = 3.1415926535898
+    for (i, j, k) in [(0, 0, 0), (2, 1, 1), (1, 0, 1)]:
+        kk = [i / xr[0] / cl[0], j / xr[1] / cl[1], k / xr[2] / cl[2]]
+        S = np.prod([cl[d] * np.exp(-PI * kk[d] ** 2) for d in range(3)])
+        assert abs(A[i, j, k] - np.sqrt(S) * np.cos(2 * PI * phi[i, j, k])) < 1e-15
+    assert np.all(A[3:] == 0) and np.all(A[:, 2:] == 0) and np.all(A[:, :, 2:] == 0)
+    # k = 0 gives S = prod corrL
+    assert abs(A[0, 0, 0] - np.sqrt(np.prod(cl)) * np.cos(2 * PI * phi[0, 0, 0])) < 1e-15
+
+
+def test_mode_r_generator_equals_scipy_dctn():
+    import scipy.fft as sf
+    Np = (12, 10, 9)
+    phi = np.random.default_rng(2).random((6, 5, 4))
+    out = O.gen_std_gauss_fft(Np, (11.0, 9.0, 8.0), (2.0, 2.0, 1.0), 1, phi)
+    A = O.mode_r_amplitude(Np, (6, 5, 4), (11.0, 9.0, 8.0), (2.0, 2.0, 1.0), 1, phi)
+    np.testing.assert_allclose(out, sf.dctn(A, type=3), atol=1e-12 * np.abs(out).max())
+    with pytest.raises(ValueError):
+        O.gen_std_gauss_fft(Np, (11.0, 9.0, 8.0), (2.0, 2.0, 1.0), 2, phi)
+
+
+def test_mode_r_mpi_support():
+    L = (8, 6, 5)
+    phi = np.random.default_rng(4).random((4, 3, 5))
+    a = O.mode_r_amplitude(L, (4, 3, 5), (7.0, 5.0, 4.0), (1.0, 1.0, 1.0), 1, phi)
+    assert np.all(a[:, :, 4] != 0) or np.any(a[:, :, 4] != 0)
+    import scipy.fft as sf
+    np.testing.assert_allclose(O.gen_std_gauss_fft_mpi(L, (7.0, 5.0, 4.0), (1.0, 1.0, 1.0), 1, phi),
+                               sf.dctn(a, type=3), atol=1e-12 * 100)
+
+
+# ---- special functions and spectra -------------------------------------------------------
+def kv_tol(nu, x):
+    """Accuracy of the ikv scheme: its series (x <= 9) cancels catastrophically towards x = 9
+    for fractional orders, its 14-term asymptotic series (x > 9) is ~1e-9 near the switch."""
+    if 2 < x <= 9:
+        return 1e-7
+    if 9 < x < 20:
+        return 2e-9
+    return 2e-12
+
+
+def test_kv_vs_scipy():
+    for i, nu in enumerate(G["kv_nu"]):
+        for j, x in enumerate(G["kv_x"]):
+            ref = G["kv"][i, j]
+            assert abs(O.kv(nu, x) - ref) <= kv_tol(nu, x) * ref, (nu, x)
+
+
+def test_kv_closed_forms():
+    for x in (0.01, 0.7, 2.0, 5.0, 12.0, 30.0):
+        k12 = np.sqrt(np.pi / (2 * x)) * np.exp(-x)
+        assert abs(O.kv(0.5, x) - k12) < max(kv_tol(0.5, x), 1e-13) * k12
+        k32 = k12 * (1 + 1 / x)
+        assert abs(O.kv(1.5, x) - k32) < max(kv_tol(1.5, x), 2e-13) * k32
+
+
+@pytest.mark.skipif(O.ref_lib() is None, reason="reference sources not compiled (oracle/_ref)")
+def test_kv_matches_reference_ikv():
+    import ctypes as C
+    R = O.ref_lib()
+    for nu in (0.1, 0.3, 1.3, 2.5):
+        for x in (0.05, 1.0, 4.0, 9.5, 25.0):
+            bk = np.zeros(8)
+            R.ref_ikv(nu, x, max(1, int(nu)), bk.ctypes.data_as(C.c_void_p))
+            ref = bk[int(nu)]
+            assert abs(O.kv(nu, x) - ref) <= kv_tol(nu, x) * abs(ref), (nu, x)
+
+
+def test_gamma_vs_scipy():
+    for x, g in zip(G["gamma_x"], G["gamma"]):
+        assert abs(O.gamma(x) - g) < 2e-14 * g
+
+
+def test_spectra_closed_forms():
+    L = (3.0, 2.0, 5.0)
+    k = (0.2, -0.4, 0.1)
+    rho2 = sum((k[d] * L[d]) ** 2 for d in range(3))
+    # exponential == Matern nu = 1/2 in 3D and 2D
+    assert abs(O.spectrum(3, 3, k, L) - O.spectrum(2, 3, k, L, 0.5)) < 1e-14 * O.spectrum(3, 3, k, L)
+    assert abs(O.spectrum(3, 2, k, L) - O.spectrum(2, 2, k, L, 0.5)) < 1e-14 * O.spectrum(3, 2, k, L)
+    assert abs(O.spectrum(3, 3, k, L) - 30.0 / (np.pi ** 2 * (1 + rho2) ** 2)) < 1e-14
+    # Gaussian is separable with the 1D pair exp(-(r/L)^2) <-> L/(2 sqrt pi) exp(-k^2 L^2/4)
+    s = np.prod([L[d] / (2 * np.sqrt(np.pi)) * np.exp(-k[d] ** 2 * L[d] ** 2 / 4) for d in range(3)])
+    assert abs(O.spectrum(1, 3, k, L) - s) < 1e-15
+
+
+def test_spectrum_integrates_to_variance():
+    # sum S dkappa over the grid -> C(0) = 1 minus the spectral mass beyond the Nyquist square.
+    # 2D Matern tail beyond radius R: 1/(1+R^2)^nu (exponential: nu = 1/2).  The square
+    # [-pi L, pi L]^2 contains the disk of radius pi L and lies inside the disk of radius sqrt2 pi L.
+    n = (256, 256, 1)
+    dx = (1.0, 1.0, 1.0)
+    dk = (2 * np.pi / 256) ** 2
+    assert abs(O.spectrum_sum(n, dx, 1, (8.0, 6.0, 1.0)) * dk - 1.0) < 1e-12
+    for model, Lc, nu in ((3, 4.0, 0.5), (3, 16.0, 0.5), (2, 5.0, 1.5), (2, 3.0, 0.3)):
+        miss = 1.0 - O.spectrum_sum(n, dx, model, (Lc, Lc, 1.0), nu) * dk
+        R = np.pi * Lc
+        lo, hi = (1 + 2 * R * R) ** -nu, (1 + R * R) ** -nu
+        assert lo * 0.999 <= miss <= hi * 1.001, (model, Lc, nu, miss, lo, hi)
+
+
+def test_target_corr_closed_forms():
+    L = (2.0, 2.0, 2.0)
+    assert O.target_corr(2, 3, (0, 0, 0), L, 0.3) == 1.0
+    r = (1.0, 0.5, 0.0)
+    rho = np.sqrt(0.25 + 0.0625)
+    assert abs(O.target_corr(2, 3, r, L, 0.5) - np.exp(-rho)) < 1e-14
+    assert abs(O.target_corr(3, 3, r, L) - np.exp(-rho)) < 1e-15
+    assert abs(O.target_corr(1, 3, r, L) - np.exp(-rho ** 2)) < 1e-15
+    # small-lag expansion of the Matern form, 0 < nu < 1: 1 - Gamma(1-nu)/Gamma(1+nu) (rho/2)^(2 nu)
+    import scipy.special as sp
+    for nu in (0.2, 0.3):  # the next term is O(rho^(2 - 2 nu)) relative
+        rho = 1e-6 / 2.0
+        exp = 1 - sp.gamma(1 - nu) / sp.gamma(1 + nu) * (rho / 2) ** (2 * nu)
+        assert abs(O.target_corr(2, 3, (1e-6, 0, 0), L, nu) - exp) < 1e-7 * (1 - exp)
+
+
+# ---- mode H generator --------------------------------------------------------------------
+def test_mode_h_equals_irfftn_of_amp_times_noise():
+    n = (16, 12, 10)
+    W = O.noise_hermitian(n, 9, 0)
+    amp = O.amplitude_h(n, (1, 1, 1), 2, (4.0, 3.0, 2.0), 0.3)
+    out = O.gen_mode_h(n, (1, 1, 1), 2, (4.0, 3.0, 2.0), 0.3, W=W)
+    ref = np.transpose(np.fft.irfftn(np.transpose(amp * W, (2, 1, 0)), s=(10, 12, 16), axes=(0, 1, 2)),
+                       (2, 1, 0)) * (16 * 12 * 10)
+    np.testing.assert_allclose(out, ref, atol=1e-12 * np.abs(ref).max())
+
+
+def test_mode_h_normalisation_sum():
+    n = (32, 24, 16)
+    amp = O.amplitude_h(n, (1, 1, 1), 2, (4.0, 3.0, 2.0), 0.3, sigma=1.7)
+    mult = np.full(amp.shape, 2.0)
+    mult[0] = 1.0
+    mult[16] = 1.0
+    assert abs(np.sum(mult * amp ** 2) - 1.7 ** 2) < 1e-12
+
+
+def test_mode_h_variance_in_band():
+    n = (64, 64, 64)
+    f = O.gen_mode_h(n, (1, 1, 1), 3, (4.0, 4.0, 4.0), seed=3)
+    # E[var] = 1 exactly for the normalised spectrum; one realisation of a field with
+    # correlation length 4 in 64^3 has var(var) small
+    assert abs(f.var() + f.mean() ** 2 - 1.0) < 0.15
+
+
+# ---- marginal, PoU, moments --------------------------------------------------------------
+def test_marginal_lognormal_closed_form():
+    mu, sd = O.marginal_params(2, 1.0, 0.3)
+    assert abs(sd ** 2 - np.log(1.09)) < 1e-15 and abs(mu - (-np.log(1.09) / 2)) < 1e-15
+    x = np.random.default_rng(0).standard_normal(2_000_000)
+    y = O.marginal(x, 2, 1.0, 0.3)
+    assert abs(y.mean() - 1.0) < 2e-3 and abs(y.std() / y.mean() - 0.3) < 3e-3
+    z = O.marginal(x[:10], 1, 2.0, 3.0)
+    np.testing.assert_allclose(z, x[:10] * 3 + 2, atol=1e-15)
+    with pytest.raises(ValueError):
+        O.marginal(x[:3], 2, -1.0, 0.3)
+
+
+def test_pou_partition_of_unity():
+    for nov in (2, 5, 16, 21):
+        lo = O.pou_weights_1d(40, nov, nov, True, False)[:nov]
+        hi = O.pou_weights_1d(40, nov, nov, False, True)[-nov:]
+        np.testing.assert_allclose(lo + hi, 1.0, atol=1e-15)
+        assert lo[0] == 0.0 and lo[-1] == 1.0 or abs(lo[-1] - 1.0) < 1e-15
+    # Nov < 2 is the identity (SRC_F/main_ScaRL.f90:1255)
+    np.testing.assert_array_equal(O.pou_weights_1d(10, 1, 1, True, True), 1.0)
+
+
+def test_pou_brute_force_topologies():
+    rng = np.random.default_rng(5)
+    Np, Nov = (10, 9, 8), (4, 3, 2)
+    for shape in ((2, 1, 1), (2, 2, 1), (2, 2, 2)):
+        # sum over owners of (sqrt w)^2 == 1 on the global grid
+        tot = {}
+        for pos in np.ndindex(*shape):
+            f = O.apply_pou(np.ones(Np), Nov, pos, shape)
+            start = [(Np[d] - Nov[d]) * pos[d] for d in range(3)]
+            for idx in np.ndindex(*Np):
+                g = tuple(start[d] + idx[d] for d in range(3))
+                tot[g] = tot.get(g, 0.0) + f[idx] ** 2
+        assert max(abs(v - 1.0) for v in tot.values()) < 1e-14
+
+
+def test_moments_kahan_vs_naive():
+    rng = np.random.default_rng(11)
+    x = 1e8 + rng.standard_normal((400_000, 2)) * 1e-3
+    ea, es, ga, gs = O.moments(x)
+    exact_mean = np.array([np.mean(np.float128(x[:, e])) if hasattr(np, "float128") else x[:, e].mean()
+                           for e in range(2)])
+    np.testing.assert_allclose(ea, exact_mean, rtol=1e-15)
+    ea2, _, _, _ = O.moments(x, naive=True)
+    assert np.all(np.abs(ea - exact_mean) <= np.abs(ea2 - exact_mean) + 1e-16)
+
+
+def test_lag_sums_brute_force():
+    x = np.asfortranarray(np.random.default_rng(2).standard_normal((5, 4, 3)))
+    s, c = O.lag_sums(x, 3)
+    assert abs(s[0, 1] - np.sum(x[:-1] * x[1:])) < 1e-13 and c[0, 1] == 4 * 4 * 3
+    assert abs(s[2, 2] - np.sum(x[:, :, :-2] * x[:, :, 2:])) < 1e-13 and c[2, 2] == 5 * 4 * 1
+
+
+# ---- Shinozuka build_Sk_mtx --------------------------------------------------------------
+def test_build_sk_shape_and_zero_planes():
+    sz = O.build_sk_shape(1, (3.0, 3.0, 3.0), (16, 16, 16), (32, 32, 32))
+    sk = O.build_sk(1, (3.0, 3.0, 3.0), (16, 16, 16), (32, 32, 32), 7)
+    assert sk.shape == (sz[0], 2 * sz[1], 2 * sz[2])
+    assert np.all(sk[0] == 0) and np.all(sk[:, 0] == 0) and np.all(sk[:, :, 0] == 0)
+    assert np.all(sk[:, -1] == 0) and np.all(sk[:, :, -1] == 0)
+    # phase magnitude is float-rounded cos/sin (cmplx without KIND, spectra_ScaRL.f90:80)
+    dk = (1 / (3.0 / 15)) / 16
+    de = (3.1415926535898 ** 2 / 2) * dk ** 2
+    v = np.sqrt(np.exp(-(1 + 1 + 1) * de)) * np.sqrt(8.0) * 2 * np.sqrt(dk ** 3)
+    assert abs(abs(sk[1, 1, 1]) - v) < 1e-6 * v
