@@ -1,0 +1,215 @@
+"""Throughput of the spectral random-field generator on one MI355X.
+
+Headline (value): batched FP32 Hermitian generator, 3D lognormal von Karman fields
+(512^3, L = 32 cells, nu = 0.3, lognormal mean 1 CoV 0.3), B realisations per step with device
+Philox white noise, replayed as one hipGraph: Gpoints/s of generated output points.
+A step = one generation of the B fields (spectrum + noise, z/y/x passes, marginal, moments).
+
+Also reported on the same line:
+  plain_fp64        -- the same spec, one FP64 field per launch (the plain drop-in path)
+  dropin_redft01    -- the reference-parity REDFT01 drop-in gen_std_gauss_FFT on 256^3 (FP64)
+  roofline          -- dominant kernel: algorithmic bytes per launch / average duration (HIP
+                       events on the library stream), against 8 TB/s
+  cpu_baseline      -- the FP64 oracle (C port, OpenMP) on one field of the same workload; that
+                       field also gives the FP32-vs-FP64 relative L2 error on shared noise.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "Gpoints/s per GPU (3D FP32 GRF), batched vs plain FP64; % HBM roofline; L2 err"
+
+# algorithmic HBM bytes per output point of each pass (FP32, device noise; the half spectrum
+# holds (n1/2+1) n2 n3 complex ~ N/2 complex = 4 B per output point)
+def pass_bytes(n, prec_bytes):
+    n1, n2, n3 = n
+    H = (n1 // 2 + 1) * n2 * n3
+    c = 2 * prec_bytes
+    N = n1 * n2 * n3
+    return {"h_zpass": H * c,            # writes the half spectrum (noise generated on chip)
+            "h_ypass": 2 * H * c,        # reads + writes it
+            "h_xpass": H * c + N * prec_bytes}  # reads it, writes the real field
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--batch", type=int, default=4)
+    ap.add_argument("--edge", type=int, default=512)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    from scarl_amd import _capi as A
+    from scarl_amd import generator as Gm
+
+    A.call("scarl_hip_init", 0)
+    lib = A.lib()
+    e = args.edge
+    n = (e, e, e)
+    L = (32.0, 32.0, 32.0)
+    B = args.batch
+    specs = [Gm.FieldSpec(n=n, model=A.CM_VON_KARMAN, corrL=L, nu=0.3, seed=2024, stream=i, margi=A.FOM_LOGNORMAL,
+                          avg=1.0, std_dev=0.3) for i in range(B)]
+
+    # ---- headline: batched FP32, hipGraph ------------------------------------------------
+    log(f"bench: batched fp32 {B} x {e}^3")
+    batch = Gm.Batch.ensemble(specs, A.PREC_FP32)
+    for _ in range(max(args.warmup, 1)):
+        batch.run(use_graph=True)
+    batch.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        batch.run(use_graph=True)
+    batch.sync()
+    dt = (time.perf_counter() - t0) / args.steps
+    pts = B * e ** 3
+    value = pts / dt / 1e9
+    log(f"bench: {dt * 1e3:.3f} ms/step, {value:.1f} Gpts/s")
+
+    # per-kernel durations with HIP events on the library stream (eager launches)
+    Gm.timing(True)
+    Gm.timing_reset()
+    for _ in range(max(3, min(args.steps, 10))):
+        batch.run(use_graph=False)
+    rep = Gm.timing_report()
+    Gm.timing(False)
+    pb = pass_bytes(n, 4)
+    passes = {}
+    for k, (cnt, tot) in rep.items():
+        avg = tot / cnt
+        byts = pb.get(k)
+        passes[k] = {"avg_ms": avg, "launches": cnt}
+        if byts is not None:
+            gbs = B * byts / (avg * 1e-3) / 1e9
+            passes[k].update({"alg_bytes": B * byts, "GBps": gbs, "frac": gbs / HBM_PEAK_GBS})
+    dom = max((k for k in passes if k in pb), key=lambda k: passes[k]["avg_ms"])
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tfile):
+        try:
+            traffic = json.load(open(tfile)).get(dom)
+        except Exception:
+            traffic = None
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": passes[dom]["GBps"], "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": passes[dom]["frac"], "traffic": traffic,
+                "bytes_per_launch": passes[dom]["alg_bytes"], "avg_ms": passes[dom]["avg_ms"]}
+    step_bytes = sum(pb.values()) * B
+    step_frac = step_bytes / dt / 1e9 / HBM_PEAK_GBS
+    log(f"bench: passes {json.dumps(passes)}")
+
+    # keep field 0 for the error check
+    f32_field0 = batch.output(0) if not args.no_cpu else None
+    moments = batch.moments()
+    batch.close()
+
+    # ---- plain FP64, one field per launch ------------------------------------------------
+    log("bench: plain fp64, one field per launch")
+    plain = Gm.Batch.ensemble(specs[:1], A.PREC_FP64)
+    plain.run()
+    plain.sync()
+    ks = max(2, args.steps // 2)
+    t0 = time.perf_counter()
+    for _ in range(ks):
+        plain.run()
+    plain.sync()
+    dtp = (time.perf_counter() - t0) / ks
+    plain_value = e ** 3 / dtp / 1e9
+    plain.close()
+
+    # ---- reference-parity REDFT01 drop-in, 256^3 FP64, inputs resident --------------------
+    log("bench: REDFT01 drop-in 256^3 fp64")
+    hip = C.CDLL("libamdhip64.so")
+    dptr = C.c_void_p()
+    rc = hip.hipMalloc(C.byref(dptr), C.c_size_t(256 ** 3 * 8))
+    if rc != 0:
+        raise RuntimeError("hipMalloc failed")
+    Np = np.array([256, 256, 256], np.int32)
+    xr = np.full(3, 255.0)
+    cl = np.array([16.0, 16.0, 8.0])
+    A.call("scarl_hip_set_marginal", 2, 1.0, 0.3)
+
+    def redft():
+        A.check(lib.scarl_hip_gen_std_gauss_fft_dev(dptr, A.ptr(Np), A.ptr(xr), A.ptr(cl), 1, 7, 0, None, 1))
+
+    redft()
+    A.check(lib.scarl_hip_batch_sync(None))
+    t0 = time.perf_counter()
+    for _ in range(ks):
+        redft()
+    A.check(lib.scarl_hip_batch_sync(None))
+    dtr = (time.perf_counter() - t0) / ks
+    hip.hipFree(dptr)
+
+    # ---- CPU baseline (oracle, FP64) and FP32 error on shared noise ----------------------
+    cpu = None
+    err = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        log(f"bench: CPU oracle, one {e}^3 field, {threads} threads")
+        t0 = time.perf_counter()
+        ref = O.gen_mode_h(n, (1, 1, 1), A.CM_VON_KARMAN, L, 0.3, seed=2024, stream=0, margi=A.FOM_LOGNORMAL,
+                           avg=1.0, std=0.3, nthreads=threads)
+        tc = time.perf_counter() - t0
+        cpu = {"value": e ** 3 / tc / 1e9, "unit": "Gpoints/s", "cores": threads, "kind": "port",
+               "sample": f"one {e}^3 lognormal von Karman field (same spec and Philox noise as field 0), "
+                         f"FP64 oracle C port, {tc:.2f} s"}
+        err = {"rel_l2": float(np.linalg.norm(f32_field0 - ref) / np.linalg.norm(ref)),
+               "max_abs": float(np.abs(f32_field0.astype(np.float64) - ref).max())}
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Gpoints/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic: device Philox4x32-10 Hermitian white noise",
+        "config": {"workload": f"batched Hermitian C2R, {B} x {e}^3 lognormal von Karman (L=32 cells, nu=0.3, "
+                               f"mean 1, CoV 0.3), one hipGraph per step",
+                   "fields_per_step": B, "edge": e, "precision": "fp32 butterflies, fp64-rounded-once twiddles"},
+        "roofline": roofline,
+        "step_roofline_frac": round(step_frac, 4),
+        "alg_bytes_per_point": sum(pb.values()) / e ** 3,
+        "passes": passes,
+        "plain_fp64": {"value": round(plain_value, 3), "unit": "Gpoints/s", "ms_per_field": round(dtp * 1e3, 4),
+                       "workload": f"one {e}^3 field per launch, FP64"},
+        "batched_vs_plain": round(value / plain_value, 3),
+        "dropin_redft01": {"value": round(256 ** 3 / dtr / 1e9, 3), "unit": "Gpoints/s",
+                           "ms_per_field": round(dtr * 1e3, 4),
+                           "workload": "gen_std_gauss_FFT 256^3 FP64 + lognormal, device phi, resident"},
+        "fp32_vs_fp64": err,
+        "moments_field0": {"mean": float(moments[0, 3]), "std": float(moments[0, 4])},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

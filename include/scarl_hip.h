@@ -64,6 +64,21 @@ int scarl_hip_gen_std_gauss_fft_dev(double* d_out, const int32_t Np[3], const do
 /* marginal fused by flags bit 0 (multiVariateTransformation, SRC_F/main_ScaRL.f90:1355-1395) */
 int scarl_hip_set_marginal(int32_t margiFirst, double avg, double std_dev);
 
+/* ---- spectra_ScaRL drop-in --------------------------------------------------------------- */
+/* build_Sk_mtx (SRC_F/spectra_ScaRL.f90:10-142) allocates Sk_mtx inside; the C side splits it
+ * into a shape query and a fill.  out_shape = (sz_Sk1, 2 sz_Sk2, 2 sz_Sk3). */
+int scarl_hip_build_sk_query(int32_t corrMod, const double xRange[3], const double corrL[3], const int32_t Np[3],
+                             const int32_t sz_c[3], int32_t seed, int32_t rank, int32_t out_shape[3]);
+/* h_sk: complex (re, im) doubles of out_shape, Fortran order.  Phases from the bit-exact LCG
+ * (seed + 1000000), rounded to single precision as cmplx() without KIND does (:80). */
+int scarl_hip_build_sk(int32_t corrMod, const double xRange[3], const double corrL[3], const int32_t Np[3],
+                       const int32_t sz_c[3], int32_t seed, int32_t rank, double* h_sk);
+/* gen_std_gauss_Shino_FFT (SRC_F/fftw_ScaRL.f90:157-255; the reference driver does not call it):
+ * Sk quadrants on the 2^ceil(log2(2 Np)) padded grid, 3D C2C backward, real part of the first Np. */
+int scarl_hip_gen_std_gauss_shino_fft(double* h_out, const int32_t Np[3], const double xRange[3],
+                                      const double corrL[3], const int32_t pointsPerCorrL[3], int32_t corrMod,
+                                      int32_t seed, int32_t rank);
+
 /* ---- normal_ScaRL drop-in (bit-exact Park-Miller LCG by jump-ahead) ---------------------- */
 /* r8_uniform_01 (SRC_F/normal_ScaRL.f90:746-841) */
 int scarl_hip_r8_uniform_01(int32_t* seed, double* r);

@@ -95,6 +95,122 @@ __global__ __launch_bounds__(kBlock) void k_r_pass(const FieldR* __restrict__ fp
 }
 
 
+
+// ------------------------------------------------------------------------------------------
+// build_Sk_mtx (SRC_F/spectra_ScaRL.f90:10-142), Gaussian branch, FP64.
+// Element t = i + s1 (j + s2 k) of the (s1, 2 sz2, 2 sz3) matrix: phase r_t from the LCG
+// jump-ahead seed' * 16807^(t+1) (r8vec_uniform_01(size, seed+1000000), :78), e^{2 PI i r} rounded
+// to single precision (cmplx without KIND, :80), planes 1 and the last y/z planes zeroed
+// (:81-85); for ii >= 2 the mirrored quadrants (:99-113) take sqrt(exp(-sum (n_d-1)^2 de_d)) of
+// their source index; everything times sqrt(8) and 2 sqrt(prod dk) (:116-117).
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t mulmod_m31_r(uint64_t a, uint64_t b) {
+    const uint64_t m = 2147483647ull;
+    uint64_t p = a * b;
+    p = (p & m) + (p >> 31);
+    p = (p & m) + (p >> 31);
+    return p >= m ? p - m : p;
+}
+__device__ __forceinline__ uint64_t powmod_m31_r(uint64_t base, uint64_t e) {
+    uint64_t r = 1;
+    while (e) {
+        if (e & 1) r = mulmod_m31_r(r, base);
+        base = mulmod_m31_r(base, base);
+        e >>= 1;
+    }
+    return r;
+}
+
+struct SkParams {
+    int s1, sz2, sz3;      // matrix is (s1, 2 sz2, 2 sz3)
+    int gaussian;          // 1: Gaussian branch; 0: phases only (von Karman branch is commented out)
+    int32_t seed;          // seed + 1000000
+    double de[3];          // (PI^2/2) dk_d^2
+    double scale;          // 2 sqrt(prod dk) (applied after sqrt(8))
+};
+
+__global__ void k_build_sk(SkParams P, cplx<double>* __restrict__ sk) {
+    const int s2 = 2 * P.sz2, s3 = 2 * P.sz3;
+    const int64_t tot = (int64_t)P.s1 * s2 * s3;
+    const int64_t s0 = ((int64_t)P.seed % 2147483647ll + 2147483647ll) % 2147483647ll;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < tot; t += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(t % P.s1);
+        const int64_t r = t / P.s1;
+        const int j = (int)(r % s2), k = (int)(r / s2);
+        const uint64_t sd = mulmod_m31_r((uint64_t)s0, powmod_m31_r(16807ull, (uint64_t)(t + 1)));
+        const double u = (double)(int32_t)sd * 4.656612875e-10;
+        const double th = 2.0 * kPiRef * u;
+        double re = (double)(float)cos(th), im = (double)(float)sin(th);
+        if (i == 0 || j == 0 || k == 0 || j == s2 - 1 || k == s3 - 1) re = im = 0.0;
+        if (P.gaussian) {
+            const int ii = i + 1, jj1 = j + 1, kk1 = k + 1;  // 1-based
+            int jj = -1, kk = -1;
+            if (jj1 >= 2 && jj1 <= P.sz2) jj = jj1;
+            else if (jj1 >= P.sz2 + 1 && jj1 <= s2 - 1) jj = s2 - jj1 + 1;
+            if (kk1 >= 2 && kk1 <= P.sz3) kk = kk1;
+            else if (kk1 >= P.sz3 + 1 && kk1 <= s3 - 1) kk = s3 - kk1 + 1;
+            if (ii >= 2 && jj >= 2 && kk >= 2) {
+                const double a = (double)((ii - 1) * (ii - 1)) * P.de[0] + (double)((jj - 1) * (jj - 1)) * P.de[1] +
+                                 (double)((kk - 1) * (kk - 1)) * P.de[2];
+                const double v = sqrt(exp(-a));
+                re *= v;
+                im *= v;
+            }
+            const double r8 = sqrt(8.0);
+            re = P.scale * (r8 * re);
+            im = P.scale * (r8 * im);
+        }
+        sk[t] = {re, im};
+    }
+}
+
+// Shinozuka consumer (SRC_F/fftw_ScaRL.f90:157-255): place the y/z quadrants of Sk into the
+// zero-padded (c1, c2, c3) grid (x entries 1..sz1/2, :217-233).
+struct ShinoPlace {
+    int s1, s2, s3;        // Sk extents
+    int hx, h2, h3;        // copied x extent (s1/2), half y, half z
+    int c1, c2, c3;        // padded grid
+};
+__global__ void k_shino_place(ShinoPlace P, const cplx<double>* __restrict__ sk, cplx<double>* __restrict__ c) {
+    const int64_t tot = (int64_t)P.c1 * P.c2 * P.c3;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < tot; t += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(t % P.c1);
+        const int64_t r = t / P.c1;
+        const int jd = (int)(r % P.c2), kd = (int)(r / P.c2);
+        cplx<double> v = {0.0, 0.0};
+        int j = -1, k = -1;
+        if (jd < P.h2) j = jd;
+        else if (jd >= P.c2 - (P.s2 - P.h2)) j = jd - (P.c2 - P.s2);
+        if (kd < P.h3) k = kd;
+        else if (kd >= P.c3 - (P.s3 - P.h3)) k = kd - (P.c3 - P.s3);
+        if (i < P.hx && j >= 0 && k >= 0) v = sk[i + (int64_t)P.s1 * (j + (int64_t)P.s2 * k)];
+        c[t] = v;
+    }
+}
+
+// x pass of the padded C2C backward transform; real part of the first (np1, np2, np3) points.
+__global__ __launch_bounds__(kBlock) void k_c2c_xcrop(const PlanDev* __restrict__ plans, int plan,
+                                                       const cplx<double>* __restrict__ pool,
+                                                       const cplx<double>* __restrict__ c, int c1, int c2,
+                                                       int np1, int np2, int np3, double* __restrict__ out) {
+    extern __shared__ char smem[];
+    cplx<double>* lds = reinterpret_cast<cplx<double>*>(smem);
+    const PlanDev& pl = plans[plan];
+    const int64_t nrows = (int64_t)np2 * np3;
+    const int64_t r0 = (int64_t)blockIdx.x * pl.T;
+    const int npen = (int)min((int64_t)pl.T, nrows - r0);
+    auto rowbase = [&](int p) -> int64_t {
+        const int64_t P = r0 + p;
+        const int64_t j = P % np2, k = P / np2;
+        return (j + (int64_t)c2 * k) * c1;
+    };
+    auto gld = [&](int p, int n) -> cplx<double> { return c[rowbase(p) + n]; };
+    auto gst = [&](int p, int n, cplx<double> v) {
+        if (n < np1) out[n + (int64_t)np1 * (r0 + p)] = v.x;
+    };
+    pencil_fft<double, LAYOUT_BF, false, true>(pl, pool, lds, npen, gld, gst);
+}
+
 hipError_t launch_r_pass(int axis, const FieldR* f, const PlanDev* plans, const void* pool, int nblocks,
                          size_t lds_bytes, hipStream_t s) {
     const cplx<double>* p = static_cast<const cplx<double>*>(pool);
@@ -105,4 +221,30 @@ hipError_t launch_r_pass(int axis, const FieldR* f, const PlanDev* plans, const 
     return hipGetLastError();
 }
 
+
+hipError_t launch_build_sk(const void* params, void* sk, int64_t tot, hipStream_t s) {
+    const SkParams P = *static_cast<const SkParams*>(params);
+    const int64_t nb = (tot + 255) / 256 < 8192 ? (tot + 255) / 256 : 8192;
+    if (tot <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_build_sk, dim3((unsigned)nb), dim3(256), 0, s, P, static_cast<cplx<double>*>(sk));
+    return hipGetLastError();
+}
+hipError_t launch_shino_place(const int* dims9, const void* sk, void* c, hipStream_t s) {
+    ShinoPlace P = {dims9[0], dims9[1], dims9[2], dims9[3], dims9[4], dims9[5], dims9[6], dims9[7], dims9[8]};
+    const int64_t tot = (int64_t)P.c1 * P.c2 * P.c3;
+    const int64_t nb = (tot + 255) / 256 < 8192 ? (tot + 255) / 256 : 8192;
+    hipLaunchKernelGGL(k_shino_place, dim3((unsigned)nb), dim3(256), 0, s, P, static_cast<const cplx<double>*>(sk),
+                       static_cast<cplx<double>*>(c));
+    return hipGetLastError();
+}
+hipError_t launch_c2c_xcrop(const PlanDev* plans, int plan, int T, const void* pool, const void* c, int c1, int c2,
+                            int np1, int np2, int np3, double* out, size_t lds, hipStream_t s) {
+    const int64_t rows = (int64_t)np2 * np3;
+    const int64_t nb = (rows + T - 1) / T;
+    if (nb <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_c2c_xcrop, dim3((unsigned)nb), dim3(kBlock), lds, s, plans, plan,
+                       static_cast<const cplx<double>*>(pool), static_cast<const cplx<double>*>(c), c1, c2, np1, np2,
+                       np3, out);
+    return hipGetLastError();
+}
 }  // namespace scarl

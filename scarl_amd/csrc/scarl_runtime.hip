@@ -55,6 +55,22 @@ hipError_t launch_sums(const double* rf, int64_t n, int nmc, int nblocks, MomPar
 hipError_t launch_lcg_uniform(int32_t seed0, int64_t n, double* r, int32_t* seeds, hipStream_t s);
 hipError_t launch_target_corr(int model, double nu, const double* rho, int n, double* out, hipStream_t s);
 hipError_t launch_kv(double nu, const double* x, int n, double* out, hipStream_t s);
+hipError_t launch_build_sk(const void* params, void* sk, int64_t tot, hipStream_t s);
+hipError_t launch_shino_place(const int* dims9, const void* sk, void* c, hipStream_t s);
+hipError_t launch_c2c_xcrop(const PlanDev* plans, int plan, int T, const void* pool, const void* c, int c1, int c2,
+                            int np1, int np2, int np3, double* out, size_t lds, hipStream_t s);
+hipError_t launch_h_strided_f64(int axis, bool gen, const FieldH* fields, const Tile* tiles, int ntiles,
+                                const PlanDev* plans, const void* pool, size_t lds_bytes, hipStream_t s);
+
+// host mirror of the kernel parameter block of k_build_sk (scarl_k_r.hip)
+struct SkParamsHost {
+    int s1, sz2, sz3;
+    int gaussian;
+    int32_t seed;
+    double de[3];
+    double scale;
+};
+
 
 // ------------------------------------------------------------------------------------------
 // errors
@@ -185,6 +201,8 @@ static int timing_collect() {
 // planner and twiddle pools
 // ------------------------------------------------------------------------------------------
 static const long double kPiL = 3.14159265358979323846264338327950288L;
+// reference PI literal (SRC_F/constants_ScaRL.f90:5)
+static const double kPiRefHost = 3.1415926535898;
 
 // exp(+2 pi i num/den) in long double
 static void cis_ld(long long num, long long den, long double& c, long double& s) {
@@ -1291,6 +1309,154 @@ int scarl_hip_batch_destroy(scarl_batch* b) {
     if (g_ctx.ready) hipStreamSynchronize(g_ctx.stream);
     delete b;
     return 0;
+}
+
+// ---- spectra_ScaRL -----------------------------------------------------------------------
+// sz_Sk = ceiling(k_cut/delta_k) with the k_cut fix-up (SRC_F/spectra_ScaRL.f90:42-72)
+static int sk_shape(int32_t corrMod, const double xRange[3], const int32_t Np[3], const int32_t sz_c[3], int32_t sz[3],
+                    double dk[3]) {
+    if (corrMod != CM_GAUSSIAN && corrMod != CM_VON_KARMAN)
+        return fail(SCARL_E_INVALID, "build_Sk_mtx: unknown corrMod %d", corrMod);
+    double k_cut[3], k_max[3];
+    bool any_lt = false;
+    for (int d = 0; d < 3; ++d) {
+        if (Np[d] < 2 || sz_c[d] < 2) return fail(SCARL_E_INVALID, "build_Sk_mtx: Np and sz_c must be >= 2");
+        k_cut[d] = 7.355;
+        const double delta_x = xRange[d] / (double)(Np[d] - 1);
+        k_max[d] = 1.0 / delta_x;
+        dk[d] = k_max[d] / (double)(sz_c[d] / 2);
+        if (k_max[d] < k_cut[d]) any_lt = true;
+    }
+    if (any_lt)
+        for (int d = 0; d < 3; ++d)
+            if (k_max[d] > k_cut[d]) k_cut[d] = k_max[d];
+    for (int d = 0; d < 3; ++d) sz[d] = (int32_t)ceil(k_cut[d] / dk[d]);
+    return 0;
+}
+
+int scarl_hip_build_sk_query(int32_t corrMod, const double xRange[3], const double corrL[3], const int32_t Np[3],
+                             const int32_t sz_c[3], int32_t seed, int32_t rank, int32_t out_shape[3]) {
+    (void)corrL; (void)seed; (void)rank;
+    if (!xRange || !Np || !sz_c || !out_shape) return fail(SCARL_E_INVALID, "null argument");
+    int32_t sz[3];
+    double dk[3];
+    SCARL_TRY(sk_shape(corrMod, xRange, Np, sz_c, sz, dk));
+    out_shape[0] = sz[0];
+    out_shape[1] = 2 * sz[1];
+    out_shape[2] = 2 * sz[2];
+    return 0;
+}
+
+static int build_sk_device(int32_t corrMod, const double xRange[3], const int32_t Np[3], const int32_t sz_c[3],
+                           int32_t seed, void** d_sk, int32_t shape[3]) {
+    int32_t sz[3];
+    double dk[3];
+    SCARL_TRY(sk_shape(corrMod, xRange, Np, sz_c, sz, dk));
+    shape[0] = sz[0];
+    shape[1] = 2 * sz[1];
+    shape[2] = 2 * sz[2];
+    const int64_t tot = (int64_t)shape[0] * shape[1] * shape[2];
+    if (tot > (int64_t)0x7fffffff) return fail(SCARL_E_UNSUPPORTED, "Sk_mtx larger than the int32 LCG stream");
+    SCARL_TRY(scratch_get("sk", (size_t)std::max<int64_t>(1, tot) * 16, d_sk));
+    SkParamsHost P;
+    P.s1 = sz[0];
+    P.sz2 = sz[1];
+    P.sz3 = sz[2];
+    P.gaussian = corrMod == CM_GAUSSIAN;
+    P.seed = seed + 1000000;  // spectra_ScaRL.f90:78
+    for (int d = 0; d < 3; ++d) P.de[d] = ((kPiRefHost * kPiRefHost) / 2.0) * (dk[d] * dk[d]);
+    P.scale = 2.0 * sqrt(dk[0] * dk[1] * dk[2]);
+    SCARL_TRY(timed("build_sk", [&] { return launch_build_sk(&P, *d_sk, tot, g_ctx.stream); }));
+    return 0;
+}
+
+int scarl_hip_build_sk(int32_t corrMod, const double xRange[3], const double corrL[3], const int32_t Np[3],
+                       const int32_t sz_c[3], int32_t seed, int32_t rank, double* h_sk) {
+    (void)corrL; (void)rank;
+    SCARL_TRY(ensure_init());
+    if (!xRange || !Np || !sz_c || !h_sk) return fail(SCARL_E_INVALID, "null argument");
+    void* d_sk;
+    int32_t shape[3];
+    SCARL_TRY(build_sk_device(corrMod, xRange, Np, sz_c, seed, &d_sk, shape));
+    const size_t bytes = (size_t)shape[0] * shape[1] * shape[2] * 16;
+    HIP_TRY(hipMemcpyAsync(h_sk, d_sk, bytes, hipMemcpyDeviceToHost, g_ctx.stream));
+    HIP_TRY(hipStreamSynchronize(g_ctx.stream));
+    return 0;
+}
+
+// gen_std_gauss_Shino_FFT (SRC_F/fftw_ScaRL.f90:157-255, not called by the reference driver)
+int scarl_hip_gen_std_gauss_shino_fft(double* h_out, const int32_t Np[3], const double xRange_in[3],
+                                      const double corrL[3], const int32_t pointsPerCorrL[3], int32_t corrMod,
+                                      int32_t seed, int32_t rank) {
+    (void)pointsPerCorrL; (void)rank;
+    SCARL_TRY(ensure_init());
+    if (!h_out || !Np || !xRange_in || !corrL) return fail(SCARL_E_INVALID, "null argument");
+    double xr[3];
+    int32_t szc[3];
+    for (int d = 0; d < 3; ++d) {
+        if (!(corrL[d] > 0.0)) return fail(SCARL_E_INVALID, "corrL must be positive");
+        xr[d] = xRange_in[d] / corrL[d];
+        szc[d] = (int32_t)pow(2.0, ceil(log((double)(2 * Np[d])) / log(2.0)));
+    }
+    void* d_sk;
+    int32_t shape[3];
+    SCARL_TRY(build_sk_device(corrMod, xr, Np, szc, seed, &d_sk, shape));
+    const int c1 = szc[0], c2 = szc[1], c3 = szc[2];
+    const int64_t ctot = (int64_t)c1 * c2 * c3;
+    void* d_c;
+    SCARL_TRY(scratch_get("shino_c", (size_t)ctot * 16, &d_c));
+    int dims9[9] = {shape[0], shape[1], shape[2], shape[0] / 2, shape[1] / 2, shape[2] / 2, c1, c2, c3};
+    SCARL_TRY(timed("shino_place", [&] { return launch_shino_place(dims9, d_sk, d_c, g_ctx.stream); }));
+    PlanSet ps;
+    ps.fp64 = true;
+    const int pz = ps.get(c3, LAYOUT_PF, 0), py = ps.get(c2, LAYOUT_PF, 0), px = ps.get(c1, LAYOUT_BF, 0);
+    if (pz < 0 || py < 0 || px < 0) return SCARL_E_UNSUPPORTED;
+    SCARL_TRY(ps.upload());
+    FieldH f;
+    memset(&f, 0, sizeof f);
+    f.n[0] = c1; f.n[1] = c2; f.n[2] = c3;
+    f.h1 = c1;
+    f.plan_z = pz; f.plan_y = py; f.plan_x = px;
+    f.first_axis = -1;
+    f.src_mode = SRC_WORKSPACE;
+    f.ws = d_c;
+    f.mom_slot = -1;
+    std::vector<Tile> tz, ty;
+    for (int64_t p = 0; p < (int64_t)c1 * c2; p += ps.plans[pz].T) tz.push_back({0, (int)p});
+    for (int64_t p = 0; p < (int64_t)c1 * c3; p += ps.plans[py].T) ty.push_back({0, (int)p});
+    void *d_f = nullptr, *d_tz = nullptr, *d_ty = nullptr;
+    std::vector<FieldH> hf(1, f);
+    int rc = upload_vec(hf, &d_f);
+    if (!rc) rc = upload_vec(tz, &d_tz);
+    if (!rc) rc = upload_vec(ty, &d_ty);
+    void* d_out = nullptr;
+    if (!rc) rc = scratch_get("shino_out", (size_t)Np[0] * Np[1] * Np[2] * 8, &d_out);
+    const size_t lz = (size_t)ps.plans[pz].lds_elems * 16, ly = (size_t)ps.plans[py].lds_elems * 16,
+                 lx = (size_t)ps.plans[px].lds_elems * 16;
+    if (!rc) rc = timed("shino_zpass", [&] {
+        return launch_h_strided_f64(2, false, (FieldH*)d_f, (Tile*)d_tz, (int)tz.size(), ps.d_plans, ps.d_pool, lz,
+                                    g_ctx.stream);
+    });
+    if (!rc) rc = timed("shino_ypass", [&] {
+        return launch_h_strided_f64(1, false, (FieldH*)d_f, (Tile*)d_ty, (int)ty.size(), ps.d_plans, ps.d_pool, ly,
+                                    g_ctx.stream);
+    });
+    if (!rc) rc = timed("shino_xpass", [&] {
+        return launch_c2c_xcrop(ps.d_plans, px, ps.plans[px].T, ps.d_pool, d_c, c1, c2, Np[0], Np[1], Np[2],
+                                (double*)d_out, lx, g_ctx.stream);
+    });
+    if (!rc) {
+        hipError_t e = hipMemcpyAsync(h_out, d_out, (size_t)Np[0] * Np[1] * Np[2] * 8, hipMemcpyDeviceToHost,
+                                      g_ctx.stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(g_ctx.stream);
+        if (e != hipSuccess) rc = fail(SCARL_E_HIP, "shino copy-out: %s", hipGetErrorString(e));
+    } else {
+        hipStreamSynchronize(g_ctx.stream);
+    }
+    if (d_f) hipFree(d_f);
+    if (d_tz) hipFree(d_tz);
+    if (d_ty) hipFree(d_ty);
+    return rc;
 }
 
 int scarl_hip_gen_field_h(void* h_out, const scarl_field_desc* d, int32_t precision, const void* h_noise) {

@@ -303,3 +303,24 @@ def test_blend_partition_of_unity_variance(hip):
     w_lo = O.pou_weights_1d(48, 8, 8, True, False)[:8]
     w_hi = O.pou_weights_1d(48, 8, 8, False, True)[-8:]
     np.testing.assert_allclose(w_lo + w_hi, 1.0, atol=1e-15)
+
+
+# ---- spectra_ScaRL: build_Sk_mtx and the Shinozuka consumer -------------------------------
+@pytest.mark.parametrize("corrMod", [1, 2])
+def test_build_sk_mtx(hip, corrMod):
+    from scarl_amd import spectra_ScaRL
+    for Np, szc, xr, seed in (((16, 16, 16), (32, 32, 32), (3.0, 3.0, 3.0), 7), ((20, 12, 9), (64, 32, 32),
+                                                                                  (4.0, 2.5, 1.7), 123)):
+        sk = spectra_ScaRL.build_Sk_mtx(corrMod, xr, (1.0, 1.0, 1.0), Np, szc, seed, 0)
+        ref = O.build_sk(corrMod, xr, Np, szc, seed)
+        assert sk.shape == ref.shape
+        np.testing.assert_allclose(sk, ref, rtol=1e-14, atol=1e-15 * np.abs(ref).max())
+
+
+def test_shino_fft(hip):
+    from scarl_amd import spectra_ScaRL
+    Np = (20, 18, 12)
+    out = np.empty(Np, np.float64, order="F")
+    spectra_ScaRL.gen_std_gauss_Shino_FFT(out, Np, (19.0, 17.0, 11.0), (4.0, 4.0, 3.0), (5, 5, 5), 1, 3, 0)
+    ref = O.gen_std_gauss_shino_fft(Np, (19.0, 17.0, 11.0), (4.0, 4.0, 3.0), 1, 3)
+    assert rel_l2(out, ref) <= FP64_TOL
