@@ -116,8 +116,17 @@ def ref_lib():
     return _ref
 
 
+class _ArrayArg:
+    """Pointer argument that keeps its (possibly temporary) array alive during the call."""
+    __slots__ = ("array", "_as_parameter_")
+
+    def __init__(self, a):
+        self.array = a
+        self._as_parameter_ = C.c_void_p(a.ctypes.data)
+
+
 def _p(a):
-    return a.ctypes.data_as(C.c_void_p)
+    return _ArrayArg(a)
 
 
 def _i3(v):

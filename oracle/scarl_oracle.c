@@ -978,6 +978,8 @@ int oracle_gen_std_gauss_shino_fft(double *out, const int32_t Np[3], const doubl
         szc[d] = (int32_t)pow(2.0, ceil(log(2.0 * Np[d]) / log(2.0)));
     }
     if (oracle_build_sk_shape(corrMod, xr, Np, szc, sz)) return 1;
+    /* quadrants that overflow or overlap the padded grid: the reference writes out of bounds */
+    if (sz[0] / 2 > szc[0] || 2 * sz[1] > szc[1] || 2 * sz[2] > szc[2]) return 3;
     long s1 = sz[0], s2 = 2L * sz[1], s3 = 2L * sz[2];
     double *sk = (double *)malloc(sizeof(double) * 2 * (size_t)(s1 * s2 * s3));
     oracle_build_sk(corrMod, xr, Np, szc, seed, sk);

@@ -119,8 +119,18 @@ def call(name, *args):
     check(getattr(lib(), name)(*args))
 
 
+class _ArrayArg:
+    """A numpy array passed as a pointer argument; holds a reference to the array so that a
+    temporary stays alive until the C call that receives it returns."""
+    __slots__ = ("array", "_as_parameter_")
+
+    def __init__(self, a):
+        self.array = a
+        self._as_parameter_ = C.c_void_p(a.ctypes.data)
+
+
 def ptr(a):
-    """Address of a numpy array's data as a c_void_p (None passes NULL)."""
+    """Pointer argument for a numpy array (None passes NULL)."""
     if a is None:
         return None
-    return C.c_void_p(a.ctypes.data)
+    return _ArrayArg(a)

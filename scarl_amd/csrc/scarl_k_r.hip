@@ -28,7 +28,7 @@ __device__ __forceinline__ double mode_r_amp(const FieldR& f, int i, int j, int 
 template <int AXIS>
 __global__ __launch_bounds__(kBlock) void k_r_pass(const FieldR* __restrict__ fp, const PlanDev* __restrict__ plans,
                                                     const cplx<double>* __restrict__ pool) {
-    extern __shared__ char smem[];
+    HIP_DYNAMIC_SHARED(char, smem)
     cplx<double>* lds = reinterpret_cast<cplx<double>*>(smem);
     const FieldR& f = *fp;
     const PlanDev& pl = plans[f.plan[AXIS]];
@@ -193,7 +193,7 @@ __global__ __launch_bounds__(kBlock) void k_c2c_xcrop(const PlanDev* __restrict_
                                                        const cplx<double>* __restrict__ pool,
                                                        const cplx<double>* __restrict__ c, int c1, int c2,
                                                        int np1, int np2, int np3, double* __restrict__ out) {
-    extern __shared__ char smem[];
+    HIP_DYNAMIC_SHARED(char, smem)
     cplx<double>* lds = reinterpret_cast<cplx<double>*>(smem);
     const PlanDev& pl = plans[plan];
     const int64_t nrows = (int64_t)np2 * np3;

@@ -70,7 +70,7 @@ __global__ __launch_bounds__(kBlock) void k_h_strided(const FieldH* __restrict__
                                                        const Tile* __restrict__ tiles,
                                                        const PlanDev* __restrict__ plans,
                                                        const cplx<T>* __restrict__ pool, int axis) {
-    extern __shared__ char smem[];
+    HIP_DYNAMIC_SHARED(char, smem)
     cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem);
     const Tile tl = tiles[blockIdx.x];
     const FieldH& f = fields[tl.field];
@@ -108,7 +108,7 @@ __global__ __launch_bounds__(kBlock) void k_h_xpass(const FieldH* __restrict__ f
                                                      int tile0, const PlanDev* __restrict__ plans,
                                                      const cplx<T>* __restrict__ pool,
                                                      MomPartial* __restrict__ partials) {
-    extern __shared__ char smem[];
+    HIP_DYNAMIC_SHARED(char, smem)
     cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem);
     const int tix = tile0 + blockIdx.x;
     const Tile tl = tiles[tix];

@@ -1398,6 +1398,17 @@ int scarl_hip_gen_std_gauss_shino_fft(double* h_out, const int32_t Np[3], const 
         xr[d] = xRange_in[d] / corrL[d];
         szc[d] = (int32_t)pow(2.0, ceil(log((double)(2 * Np[d])) / log(2.0)));
     }
+    {
+        // the quadrant copies (fftw_ScaRL.f90:217-233) must fit the padded grid without
+        // overlapping; otherwise the reference writes out of bounds (k_max < k_cut regime)
+        int32_t sz[3];
+        double dk[3];
+        SCARL_TRY(sk_shape(corrMod, xr, Np, szc, sz, dk));
+        if (sz[0] / 2 > szc[0] || 2 * sz[1] > szc[1] || 2 * sz[2] > szc[2])
+            return fail(SCARL_E_UNSUPPORTED,
+                        "Sk quadrants (%d,%d,%d) exceed the padded grid (%d,%d,%d): mesh too coarse for this "
+                        "representation", sz[0], 2 * sz[1], 2 * sz[2], szc[0], szc[1], szc[2]);
+    }
     void* d_sk;
     int32_t shape[3];
     SCARL_TRY(build_sk_device(corrMod, xr, Np, szc, seed, &d_sk, shape));

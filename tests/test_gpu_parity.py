@@ -318,9 +318,13 @@ def test_build_sk_mtx(hip, corrMod):
 
 
 def test_shino_fft(hip):
-    from scarl_amd import spectra_ScaRL
-    Np = (20, 18, 12)
+    from scarl_amd import spectra_ScaRL, ScarlError
+    # dx = 0.125 corrL: k_max = 8 > k_cut, the quadrants fit the 64x64x32 padded grid
+    Np = (20, 18, 16)
     out = np.empty(Np, np.float64, order="F")
-    spectra_ScaRL.gen_std_gauss_Shino_FFT(out, Np, (19.0, 17.0, 11.0), (4.0, 4.0, 3.0), (5, 5, 5), 1, 3, 0)
-    ref = O.gen_std_gauss_shino_fft(Np, (19.0, 17.0, 11.0), (4.0, 4.0, 3.0), 1, 3)
+    spectra_ScaRL.gen_std_gauss_Shino_FFT(out, Np, (9.5, 8.5, 7.5), (4.0, 4.0, 4.0), (5, 5, 5), 1, 3, 0)
+    ref = O.gen_std_gauss_shino_fft(Np, (9.5, 8.5, 7.5), (4.0, 4.0, 4.0), 1, 3)
     assert rel_l2(out, ref) <= FP64_TOL
+    # too coarse a mesh: the reference would write outside c_in; here it is an error
+    with pytest.raises(ScarlError):
+        spectra_ScaRL.gen_std_gauss_Shino_FFT(out, Np, (19.0, 17.0, 15.0), (4.0, 4.0, 3.0), (5, 5, 5), 1, 3, 0)
