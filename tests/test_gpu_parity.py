@@ -62,7 +62,9 @@ def test_gen_std_gauss_fft_256_cube(hip):
     m = O.marginal(ref, 2, 1.0, 0.3)
     np.testing.assert_array_equal(np.isinf(fused), np.isinf(m))
     np.testing.assert_array_equal(fused == 0, m == 0)
-    ok = np.isfinite(m) & (m > 0)
+    # subnormal results (exponent below -708) carry too few significant bits to invert
+    tiny = np.finfo(np.float64).tiny
+    ok = np.isfinite(m) & (m >= tiny) & (fused >= tiny)
     mu, sd = O.marginal_params(2, 1.0, 0.3)
     assert rel_l2((np.log(fused[ok]) - mu) / sd, ref[ok]) <= FP64_TOL
 
