@@ -27,13 +27,14 @@ struct Ecap<double> {
 // reference PI literal (SRC_F/constants_ScaRL.f90:5)
 constexpr double kPiRef = 3.1415926535898;
 
-//   gld(p, n) -> cplx<T> : element n of tile pencil p (pass input)
-//   gst(p, n, v)         : element n of tile pencil p (pass output)
+//   ctx = prep(p)          : per-thread context of tile pencil p (index decoding done once)
+//   gld(ctx, n) -> cplx<T> : element n of the pencil (pass input)
+//   gst(ctx, n, v)         : element n of the pencil (pass output)
 // SIN: load the whole tile into LDS first (one copy of gld).  SOUT: finish the transform in
 // LDS, then run gst from one loop.
-template <typename T, int LAYOUT, bool SIN, bool SOUT, class GLoad, class GStore>
+template <typename T, int LAYOUT, bool SIN, bool SOUT, class Prep, class GLoad, class GStore>
 __device__ __forceinline__ void pencil_fft(const PlanDev& pl, const cplx<T>* __restrict__ pool, cplx<T>* lds,
-                                           int npen, GLoad&& gld, GStore&& gst) {
+                                           int npen, Prep&& prep, GLoad&& gld, GStore&& gst) {
     constexpr int EC = Ecap<T>::v;
     const int tid = threadIdx.x;
     const int Tt = pl.T, tpp = pl.tpp, N = pl.N, ldp = pl.ldp;
@@ -50,8 +51,9 @@ __device__ __forceinline__ void pencil_fft(const PlanDev& pl, const cplx<T>* __r
         if constexpr (LAYOUT == LAYOUT_PF) return n * Tt + p;
         else return p * ldp + n + (n >> 4);
     };
-    auto ld_g = [&](int n) -> cplx<T> { return gld(p, n); };
-    auto st_g = [&](int n, cplx<T> v) { gst(p, n, v); };
+    const auto ctx = prep(active ? p : 0);
+    auto ld_g = [&](int n) -> cplx<T> { return gld(ctx, n); };
+    auto st_g = [&](int n, cplx<T> v) { gst(ctx, n, v); };
     auto ld_l = [&](int n) -> cplx<T> { return lds[addr(n)]; };
     auto st_l = [&](int n, cplx<T> v) { lds[addr(n)] = v; };
 

@@ -45,53 +45,74 @@ __global__ __launch_bounds__(kBlock) void k_r_pass(const FieldR* __restrict__ fp
     const int npen = (int)min((int64_t)pl.T, npen_cplx - c0);
     const cplx<double>* pre = pool + pl.pre_off;
 
-    auto rin = [&](int64_t P, int j) -> double {
-        if (j >= m) return 0.0;
-        if constexpr (AXIS == 0) {
-            const int jy = (int)(P % m2), kz = (int)(P / m2);
-            return mode_r_amp(f, j, jy, kz);
-        } else if constexpr (AXIS == 1) {
-            const int64_t i = P % n1, k = P / n1;
-            return f.tmp1[i + (int64_t)n1 * (j + (int64_t)m2 * k)];
-        } else {
-            return f.tmp2[P + (int64_t)n1 * n2 * j];
-        }
+    // per-thread context: the (up to) two real pencils of complex pencil p, decoded once
+    struct Ctx {
+        int64_t in[2], out[2];   // element-0 offsets of input / output of each real pencil
+        int jy[2], kz[2];        // x pass: octant coordinates of the rows
+        bool hasb;
     };
-    auto rout = [&](int64_t P, int o, double v) {
+    auto prep = [&](int p) -> Ctx {
+        Ctx c;
+        const int64_t Pa = 2 * (c0 + p);
+        c.hasb = Pa + 1 < npen_real;
+        for (int e = 0; e < 2; ++e) {
+            const int64_t P = (e == 0 || c.hasb) ? Pa + e : Pa;
+            if constexpr (AXIS == 0) {
+                c.jy[e] = (int)(P % m2);
+                c.kz[e] = (int)(P / m2);
+                c.in[e] = 0;
+                c.out[e] = (int64_t)n1 * P;
+            } else if constexpr (AXIS == 1) {
+                const int64_t i = P % n1, k = P / n1;
+                c.jy[e] = c.kz[e] = 0;
+                c.in[e] = i + (int64_t)n1 * m2 * k;
+                c.out[e] = i + (int64_t)n1 * n2 * k;
+            } else {
+                c.jy[e] = c.kz[e] = 0;
+                c.in[e] = P;
+                c.out[e] = P;
+            }
+        }
+        return c;
+    };
+    auto rin = [&](const Ctx& c, int e, int j) -> double {
+        if (j >= m) return 0.0;
+        if constexpr (AXIS == 0) return mode_r_amp(f, j, c.jy[e], c.kz[e]);
+        else if constexpr (AXIS == 1) return f.tmp1[c.in[e] + (int64_t)n1 * j];
+        else return f.tmp2[c.in[e] + (int64_t)n1 * n2 * j];
+    };
+    auto rout = [&](const Ctx& c, int e, int o, double v) {
         if constexpr (AXIS == 0) {
-            f.tmp1[o + (int64_t)n1 * P] = v;
+            f.tmp1[c.out[e] + o] = v;
         } else if constexpr (AXIS == 1) {
-            const int64_t i = P % n1, k = P / n1;
-            f.tmp2[i + (int64_t)n1 * (o + (int64_t)n2 * k)] = v;
+            f.tmp2[c.out[e] + (int64_t)n1 * o] = v;
         } else {
             double r = v;
             if (f.margi == FOM_GAUSSIAN) r = r * f.sd + f.mu;
             else if (f.margi == FOM_LOGNORMAL) r = exp(r * f.sd + f.mu);
-            f.out[P + (int64_t)n1 * n2 * o] = r;
+            f.out[c.out[e] + (int64_t)n1 * n2 * o] = r;
         }
     };
-    auto gld = [&](int p, int j) -> cplx<double> {
-        const int64_t Pa = 2 * (c0 + p), Pb = Pa + 1;
+    auto gld = [&](const Ctx& c, int j) -> cplx<double> {
         const cplx<double> w = pre[j];
-        const double xa = rin(Pa, j), xa2 = (j > 0) ? rin(Pa, n - j) : 0.0;
+        const double xa = rin(c, 0, j), xa2 = (j > 0) ? rin(c, 0, n - j) : 0.0;
         double xb = 0.0, xb2 = 0.0;
-        if (Pb < npen_real) {
-            xb = rin(Pb, j);
-            xb2 = (j > 0) ? rin(Pb, n - j) : 0.0;
+        if (c.hasb) {
+            xb = rin(c, 1, j);
+            xb2 = (j > 0) ? rin(c, 1, n - j) : 0.0;
         }
         // V(a) = (xa - i xa2) w ; V(b) likewise ; return V(a) + i V(b)
         const cplx<double> va = {xa * w.x + xa2 * w.y, xa * w.y - xa2 * w.x};
         const cplx<double> vb = {xb * w.x + xb2 * w.y, xb * w.y - xb2 * w.x};
         return {va.x - vb.y, va.y + vb.x};
     };
-    auto gst = [&](int p, int k, cplx<double> v) {
-        const int64_t Pa = 2 * (c0 + p), Pb = Pa + 1;
+    auto gst = [&](const Ctx& c, int k, cplx<double> v) {
         const int o = (2 * k < n) ? 2 * k : 2 * (n - 1 - k) + 1;
-        rout(Pa, o, v.x);
-        if (Pb < npen_real) rout(Pb, o, v.y);
+        rout(c, 0, o, v.x);
+        if (c.hasb) rout(c, 1, o, v.y);
     };
-    if constexpr (AXIS == 0) pencil_fft<double, LAYOUT_BF, true, false>(pl, pool, lds, npen, gld, gst);
-    else pencil_fft<double, LAYOUT_PF, false, false>(pl, pool, lds, npen, gld, gst);
+    if constexpr (AXIS == 0) pencil_fft<double, LAYOUT_BF, true, false>(pl, pool, lds, npen, prep, gld, gst);
+    else pencil_fft<double, LAYOUT_PF, false, false>(pl, pool, lds, npen, prep, gld, gst);
 }
 
 
@@ -199,16 +220,19 @@ __global__ __launch_bounds__(kBlock) void k_c2c_xcrop(const PlanDev* __restrict_
     const int64_t nrows = (int64_t)np2 * np3;
     const int64_t r0 = (int64_t)blockIdx.x * pl.T;
     const int npen = (int)min((int64_t)pl.T, nrows - r0);
-    auto rowbase = [&](int p) -> int64_t {
+    struct Ctx {
+        int64_t in, out;
+    };
+    auto prep = [&](int p) -> Ctx {
         const int64_t P = r0 + p;
         const int64_t j = P % np2, k = P / np2;
-        return (j + (int64_t)c2 * k) * c1;
+        return {(j + (int64_t)c2 * k) * c1, (int64_t)np1 * P};
     };
-    auto gld = [&](int p, int n) -> cplx<double> { return c[rowbase(p) + n]; };
-    auto gst = [&](int p, int n, cplx<double> v) {
-        if (n < np1) out[n + (int64_t)np1 * (r0 + p)] = v.x;
+    auto gld = [&](const Ctx& x, int n) -> cplx<double> { return c[x.in + n]; };
+    auto gst = [&](const Ctx& x, int n, cplx<double> v) {
+        if (n < np1) out[x.out + n] = v.x;
     };
-    pencil_fft<double, LAYOUT_BF, false, true>(pl, pool, lds, npen, gld, gst);
+    pencil_fft<double, LAYOUT_BF, false, true>(pl, pool, lds, npen, prep, gld, gst);
 }
 
 hipError_t launch_r_pass(int axis, const FieldR* f, const PlanDev* plans, const void* pool, int nblocks,

@@ -51,11 +51,7 @@ __device__ __forceinline__ cplx<T> noise_h(const FieldH& f, int kx, int ky, int 
 }
 
 template <typename T>
-__device__ __forceinline__ cplx<T> generate_h(const FieldH& f, int64_t idx) {
-    const int h1 = f.h1, n2 = f.n[1];
-    const int kx = (int)(idx % h1);
-    const int64_t r = idx / h1;
-    const int ky = (int)(r % n2), kz = (int)(r / n2);
+__device__ __forceinline__ cplx<T> generate_h(const FieldH& f, int kx, int ky, int kz, int64_t idx) {
     cplx<T> w;
     if (f.src_mode == SRC_HOST_NOISE) w = reinterpret_cast<const cplx<T>*>(f.W)[idx];
     else w = noise_h<T>(f, kx, ky, kz, idx);
@@ -75,27 +71,31 @@ __global__ __launch_bounds__(kBlock) void k_h_strided(const FieldH* __restrict__
     const Tile tl = tiles[blockIdx.x];
     const FieldH& f = fields[tl.field];
     const PlanDev& pl = plans[axis == 2 ? f.plan_z : f.plan_y];
-    const int64_t h1 = f.h1, n2 = f.n[1], n3 = f.n[2];
-    const int64_t A = (axis == 2) ? h1 * n2 : h1;
-    const int64_t B = (axis == 2) ? 0 : h1 * n2;
-    const int64_t S = (axis == 2) ? h1 * n2 : h1;
-    const int64_t npen_total = (axis == 2) ? h1 * n2 : h1 * n3;
-    const int npen = (int)min((int64_t)pl.T, npen_total - tl.pencil0);
+    const int h1 = f.h1, n2 = f.n[1], n3 = f.n[2];
+    const int npen_total = (axis == 2) ? h1 * n2 : h1 * n3;
+    const int npen = min(pl.T, npen_total - tl.pencil0);
+    const int64_t S = (axis == 2) ? (int64_t)h1 * n2 : (int64_t)h1;
     cplx<T>* ws = reinterpret_cast<cplx<T>*>(f.ws);
     const bool gen = GEN && (f.first_axis == axis) && (f.src_mode != SRC_WORKSPACE);
-    auto base = [&](int p) -> int64_t {
-        const int64_t P = tl.pencil0 + p;
-        return (P % A) + (P / A) * B;
+    struct Ctx {
+        int64_t base;
+        int kx, k2;  // k2 = ky (z pass) or kz (y pass)
     };
-    auto gld = [&](int p, int n) -> cplx<T> {
-        const int64_t idx = base(p) + n * S;
+    auto prep = [&](int p) -> Ctx {
+        const int P = tl.pencil0 + p;
+        const int kx = P % h1, q = P / h1;
+        const int64_t base = (axis == 2) ? (int64_t)P : kx + (int64_t)h1 * n2 * q;
+        return {base, kx, q};
+    };
+    auto gld = [&](const Ctx& c, int n) -> cplx<T> {
+        const int64_t idx = c.base + n * S;
         if constexpr (GEN) {
-            if (gen) return generate_h<T>(f, idx);
+            if (gen) return (axis == 2) ? generate_h<T>(f, c.kx, c.k2, n, idx) : generate_h<T>(f, c.kx, n, c.k2, idx);
         }
         return ws[idx];
     };
-    auto gst = [&](int p, int n, cplx<T> v) { ws[base(p) + n * S] = v; };
-    pencil_fft<T, LAYOUT_PF, GEN, false>(pl, pool, lds, npen, gld, gst);
+    auto gst = [&](const Ctx& c, int n, cplx<T> v) { ws[c.base + n * S] = v; };
+    pencil_fft<T, LAYOUT_PF, GEN, false>(pl, pool, lds, npen, prep, gld, gst);
 }
 
 // x pass: half-length complex FFT of Z_k = (X_k + X*_{M-k}) + i e^{2 pi i k/n1} (X_k - X*_{M-k})
@@ -116,7 +116,7 @@ __global__ __launch_bounds__(kBlock) void k_h_xpass(const FieldH* __restrict__ f
     const PlanDev& pl = plans[f.plan_x];
     const int n1 = f.n[0], n2 = f.n[1], n3 = f.n[2], h1 = f.h1, M = n1 / 2;
     const int64_t nrows = (int64_t)n2 * n3;
-    const int npen = (int)min((int64_t)pl.T, nrows - tl.pencil0);
+    const int npen = (int)min((int64_t)pl.T, nrows - (int64_t)tl.pencil0);
     const cplx<T>* ws = reinterpret_cast<const cplx<T>*>(f.ws);
     const cplx<T>* pre = pool + pl.pre_off;
     T* out = reinterpret_cast<T*>(f.out);
@@ -124,9 +124,26 @@ __global__ __launch_bounds__(kBlock) void k_h_xpass(const FieldH* __restrict__ f
     double ms = 0.0, mc = 0.0, ms2 = 0.0, mc2 = 0.0;
     const bool mom = f.mom_slot >= 0;
 
-    auto gld = [&](int p, int k) -> cplx<T> {
-        const cplx<T>* X = ws + (tl.pencil0 + p) * (int64_t)h1;
-        cplx<T> a = X[k], bb = X[M - k];
+    struct Ctx {
+        const cplx<T>* X;
+        T* o;
+        T wyz;     // PoU weight of the row (blend)
+        int myz;   // overlap mask bits of the row (blend)
+    };
+    auto prep = [&](int p) -> Ctx {
+        const int row = tl.pencil0 + p;
+        const int ky = row % n2, kz = row / n2;
+        Ctx c = {ws + (int64_t)row * h1, out + ky * f.out_ld1 + kz * f.out_ld2, T(1), 0};
+        if (f.blend) {
+            c.wyz = pou_sqrt_weight<T>(ky, n2, f.nov_lo[1], f.nov_hi[1]) *
+                    pou_sqrt_weight<T>(kz, n3, f.nov_lo[2], f.nov_hi[2]);
+            c.myz = (in_overlap(ky, n2, f.nov_lo[1], f.nov_hi[1]) ? 2 : 0) |
+                    (in_overlap(kz, n3, f.nov_lo[2], f.nov_hi[2]) ? 4 : 0);
+        }
+        return c;
+    };
+    auto gld = [&](const Ctx& c, int k) -> cplx<T> {
+        cplx<T> a = c.X[k], bb = c.X[M - k];
         if (k == 0) {
             a.y = T(0);
             bb.y = T(0);
@@ -136,10 +153,8 @@ __global__ __launch_bounds__(kBlock) void k_h_xpass(const FieldH* __restrict__ f
         const cplx<T> wd = cmul(pre[k], d);
         return {s.x - wd.y, s.y + wd.x};
     };
-    auto gst = [&](int p, int k, cplx<T> z) {
-        const int64_t row = tl.pencil0 + p;
-        const int ky = (int)(row % n2), kz = (int)(row / n2);
-        T* o = out + ky * f.out_ld1 + kz * f.out_ld2;
+    auto gst = [&](const Ctx& c, int k, cplx<T> z) {
+        T* o = c.o;
         if (!f.blend) {
             const T r0 = apply_margi<T>(z.x, f.margi, mu, sd), r1 = apply_margi<T>(z.y, f.margi, mu, sd);
             if (mom) {
@@ -152,10 +167,8 @@ __global__ __launch_bounds__(kBlock) void k_h_xpass(const FieldH* __restrict__ f
             else *reinterpret_cast<double2*>(o + 2 * k) = make_double2(r0, r1);
             return;
         }
-        const T wyz = pou_sqrt_weight<T>(ky, n2, f.nov_lo[1], f.nov_hi[1]) *
-                      pou_sqrt_weight<T>(kz, n3, f.nov_lo[2], f.nov_hi[2]);
-        const int myz = (in_overlap(ky, n2, f.nov_lo[1], f.nov_hi[1]) ? 2 : 0) |
-                        (in_overlap(kz, n3, f.nov_lo[2], f.nov_hi[2]) ? 4 : 0);
+        const T wyz = c.wyz;
+        const int myz = c.myz;
 #pragma unroll
         for (int e = 0; e < 2; ++e) {
             const int i = 2 * k + e;
@@ -175,7 +188,7 @@ __global__ __launch_bounds__(kBlock) void k_h_xpass(const FieldH* __restrict__ f
             o[i] = val;
         }
     };
-    pencil_fft<T, LAYOUT_BF, false, true>(pl, pool, lds, npen, gld, gst);
+    pencil_fft<T, LAYOUT_BF, false, true>(pl, pool, lds, npen, prep, gld, gst);
     if (mom) {
         __syncthreads();
         block_reduce_mom(ms, mc, ms2, mc2, reinterpret_cast<double*>(smem), partials + tix);
