@@ -110,14 +110,21 @@ static void philox_at(uint64_t idx, uint64_t seed, uint32_t stream, uint32_t o[4
 /* 24-bit midpoint uniform in (0,1): exactly representable in float and double. */
 static double u24(uint32_t x) { return (double)(x >> 8) * 0x1p-24 + 0x1p-25; }
 
-/* Standard-normal pair for counter idx (Box-Muller on two 24-bit uniforms). */
-void oracle_gauss_pair(uint64_t idx, uint64_t seed, uint32_t stream, double g[2]) {
-    uint32_t o[4];
-    philox_at(idx, seed, stream, o);
-    double u1 = u24(o[0]), u2 = u24(o[1]);
+/* Box-Muller on two 24-bit uniforms. */
+static void box_muller(uint32_t a, uint32_t b, double g[2]) {
+    double u1 = u24(a), u2 = u24(b);
     double rad = sqrt(-2.0 * log(u1));
     g[0] = rad * cos(2.0 * PI_EXACT * u2);
     g[1] = rad * sin(2.0 * PI_EXACT * u2);
+}
+
+/* Standard-normal pair of half-spectrum point (kx, ky, kz): Philox counter = pair index
+ * kx + h1 (ky + n2 floor(kz/2)); outputs (0,1) feed even kz, outputs (2,3) odd kz. */
+void oracle_gauss_at(long kx, long ky, long kz, long h1, long n2, uint64_t seed, uint32_t stream, double g[2]) {
+    uint32_t o[4];
+    philox_at((uint64_t)(kx + h1 * (ky + n2 * (kz >> 1))), seed, stream, o);
+    if (kz & 1) box_muller(o[2], o[3], g);
+    else box_muller(o[0], o[1], g);
 }
 
 /* 53-bit uniform in [0,1) for counter idx (mode R phase phi). */
@@ -618,6 +625,7 @@ static long signed_freq(long m, long n) { return (m > n / 2) ? m - n : m; }
  * points are real N(0,1).  Counter = linear half-spectrum index of the canonical point. */
 void oracle_noise_hermitian(double *W, const int n[3], uint64_t seed, uint32_t stream) {
     long n1 = n[0], n2 = n[1], n3 = n[2], h1 = n1 / 2 + 1;
+    const double rs2 = 0.70710678118654752440;
     for (long kz = 0; kz < n3; ++kz)
         for (long ky = 0; ky < n2; ++ky)
             for (long kx = 0; kx < h1; ++kx) {
@@ -625,24 +633,22 @@ void oracle_noise_hermitian(double *W, const int n[3], uint64_t seed, uint32_t s
                 double g[2];
                 int selfconj_plane = (kx == 0) || ((n1 % 2 == 0) && kx == n1 / 2);
                 if (!selfconj_plane) {
-                    oracle_gauss_pair((uint64_t)idx, seed, stream, g);
-                    W[2 * idx] = g[0] * 0.70710678118654752440;
-                    W[2 * idx + 1] = g[1] * 0.70710678118654752440;
+                    oracle_gauss_at(kx, ky, kz, h1, n2, seed, stream, g);
+                    W[2 * idx] = g[0] * rs2;
+                    W[2 * idx + 1] = g[1] * rs2;
                     continue;
                 }
                 long py = (n2 - ky) % n2, pz = (n3 - kz) % n3;
                 long lin_self = ky + n2 * kz, lin_part = py + n2 * pz;
+                long cy = lin_self <= lin_part ? ky : py, cz = lin_self <= lin_part ? kz : pz;
+                oracle_gauss_at(kx, cy, cz, h1, n2, seed, stream, g);
                 if (lin_self == lin_part) {
-                    oracle_gauss_pair((uint64_t)idx, seed, stream, g);
                     W[2 * idx] = g[0];
                     W[2 * idx + 1] = 0.0;
                 } else {
-                    long canon = lin_self < lin_part ? lin_self : lin_part;
-                    long cidx = kx + h1 * canon;
-                    oracle_gauss_pair((uint64_t)cidx, seed, stream, g);
                     double s = (lin_self < lin_part) ? 1.0 : -1.0;
-                    W[2 * idx] = g[0] * 0.70710678118654752440;
-                    W[2 * idx + 1] = s * g[1] * 0.70710678118654752440;
+                    W[2 * idx] = g[0] * rs2;
+                    W[2 * idx + 1] = s * g[1] * rs2;
                 }
             }
 }

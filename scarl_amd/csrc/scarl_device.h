@@ -447,22 +447,31 @@ __device__ __forceinline__ T u24(uint32_t x) {
     return static_cast<T>(x >> 8) * static_cast<T>(5.9604644775390625e-08) + static_cast<T>(2.98023223876953125e-08);
 }
 
-// standard-normal pair (g0, g1) for spectral counter idx
+// Box-Muller on two 24-bit midpoint uniforms: (sqrt(-2 ln u1) cos 2 pi u2, sqrt(-2 ln u1) sin 2 pi u2).
+// FP32 uses the hardware transcendentals (v_log_f32 is log2; v_sin/v_cos take revolutions, so
+// sin(2 pi u2) is one instruction); FP64 uses the correctly rounded library functions.
 template <typename T>
-__device__ __forceinline__ cplx<T> gauss_pair(uint64_t idx, uint64_t seed, uint32_t stream) {
-    uint32_t o[4];
-    philox_at(idx, seed, stream, o);
-    const T u1 = u24<T>(o[0]), u2 = u24<T>(o[1]);
-    T s, c;
+__device__ __forceinline__ cplx<T> box_muller(uint32_t a, uint32_t b) {
+    const T u1 = u24<T>(a), u2 = u24<T>(b);
     if constexpr (sizeof(T) == 4) {
-        const float rad = sqrtf(-2.0f * logf(u1));
-        sincospif(2.0f * u2, &s, &c);
-        return {rad * c, rad * s};
+        const float rad = __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u1));
+        return {rad * __builtin_amdgcn_cosf(u2), rad * __builtin_amdgcn_sinf(u2)};
     } else {
+        double s, c;
         const double rad = sqrt(-2.0 * log(u1));
         sincospi(2.0 * u2, &s, &c);
         return {rad * c, rad * s};
     }
+}
+
+// Standard-normal pair of spectral point with kz parity `half`, from the Philox call whose
+// counter is the point's pair index ctr = kx + h1 (ky + n2 floor(kz/2)): outputs (0,1) feed the
+// even kz, outputs (2,3) the odd kz, so one call serves two neighbouring points.
+template <typename T>
+__device__ __forceinline__ cplx<T> gauss_at(uint64_t ctr, int half, uint64_t seed, uint32_t stream) {
+    uint32_t o[4];
+    philox_at(ctr, seed, stream, o);
+    return half ? box_muller<T>(o[2], o[3]) : box_muller<T>(o[0], o[1]);
 }
 
 // 53-bit uniform in [0,1)

@@ -11,6 +11,8 @@
 #include "scarl_device.h"
 #include "scarl_internal.h"
 
+#include <type_traits>
+
 namespace scarl {
 
 template <typename T>
@@ -30,11 +32,16 @@ constexpr double kPiRef = 3.1415926535898;
 //   ctx = prep(p)          : per-thread context of tile pencil p (index decoding done once)
 //   gld(ctx, n) -> cplx<T> : element n of the pencil (pass input)
 //   gst(ctx, n, v)         : element n of the pencil (pass output)
-// SIN: load the whole tile into LDS first (one copy of gld).  SOUT: finish the transform in
-// LDS, then run gst from one loop.
-template <typename T, int LAYOUT, bool SIN, bool SOUT, class Prep, class GLoad, class GStore>
+//   gin(ctx, b, tpp, put)  : (SIN only) writes the pencil's N inputs through put(n, v); thread b of
+//                            tpp threads; the default loops n = b, b + tpp, ... over gld
+// SIN: load the whole tile into LDS first (one copy of the loader).  SOUT: finish the transform
+// in LDS, then run gst from one loop.
+struct DefaultStageIn {};
+
+template <typename T, int LAYOUT, bool SIN, bool SOUT, class Prep, class GLoad, class GStore,
+          class GIn = DefaultStageIn>
 __device__ __forceinline__ void pencil_fft(const PlanDev& pl, const cplx<T>* __restrict__ pool, cplx<T>* lds,
-                                           int npen, Prep&& prep, GLoad&& gld, GStore&& gst) {
+                                           int npen, Prep&& prep, GLoad&& gld, GStore&& gst, GIn&& gin = GIn()) {
     constexpr int EC = Ecap<T>::v;
     const int tid = threadIdx.x;
     const int Tt = pl.T, tpp = pl.tpp, N = pl.N, ldp = pl.ldp;
@@ -80,9 +87,14 @@ __device__ __forceinline__ void pencil_fft(const PlanDev& pl, const cplx<T>* __r
         return;
     }
     if constexpr (SIN) {
+        if (active) {
+            if constexpr (std::is_same<typename std::decay<GIn>::type, DefaultStageIn>::value) {
 #pragma unroll 1
-        for (int n = b; n < N; n += tpp)
-            if (active) st_l(n, ld_g(n));
+                for (int n = b; n < N; n += tpp) st_l(n, ld_g(n));
+            } else {
+                gin(ctx, b, tpp, st_l);
+            }
+        }
         __syncthreads();
     }
     const int S = pl.nst;

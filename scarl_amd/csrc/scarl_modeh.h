@@ -18,43 +18,55 @@ __device__ __forceinline__ T amplitude_h(const FieldH& f, int kx, int ky, int kz
     const T a0 = T(f.alpha[0]) * fx, a1 = T(f.alpha[1]) * fy, a2 = T(f.alpha[2]) * fz;
     const T q = a0 * a0 + a1 * a1 + a2 * a2;
     if (f.model == CM_GAUSSIAN) {
-        if constexpr (sizeof(T) == 4) return T(f.amp0) * expf(-0.125f * q);
+        if constexpr (sizeof(T) == 4) return T(f.amp0) * __builtin_amdgcn_exp2f(-0.18033688011112042f * q);
         else return f.amp0 * exp(-0.125 * q);
     } else {
-        if constexpr (sizeof(T) == 4) return T(f.amp0) * exp2f(T(f.expo) * log2f(1.0f + q));
+        if constexpr (sizeof(T) == 4)
+            return T(f.amp0) * __builtin_amdgcn_exp2f(T(f.expo) * __builtin_amdgcn_logf(1.0f + q));
         else return f.amp0 * pow(1.0 + q, f.expo);
     }
 }
 
-// Hermitian white noise at half-spectrum linear index idx = kx + h1 (ky + n2 kz):
-// off the self-conjugate planes (kx = 0, kx = n1/2) a complex N(0,1) from the point's own counter;
-// on them W(kx,-ky,-kz) = conj W(kx,ky,kz), drawn from the counter of the lower linear index,
-// and the self-conjugate points are real N(0,1).
+// Hermitian white noise at half-spectrum point (kx, ky, kz).  Off the self-conjugate planes
+// (kx = 0, kx = n1/2) W = (g1 + i g2)/sqrt2 of the point itself; on them W(kx,-ky,-kz) =
+// conj W(kx,ky,kz), drawn at the member of the pair with the lower linear index, and the
+// self-conjugate points are real g1.  The draw of point (kx, ky, kz) is gauss_at with
+// counter kx + h1 (ky + n2 floor(kz/2)) and half = kz mod 2.
 template <typename T>
-__device__ __forceinline__ cplx<T> noise_h(const FieldH& f, int kx, int ky, int kz, int64_t idx) {
+__device__ __forceinline__ cplx<T> noise_h(const FieldH& f, int kx, int ky, int kz) {
     const T rs2 = T(0.70710678118654752440084436210484904);
     const int n1 = f.n[0], n2 = f.n[1], n3 = f.n[2];
+    const int64_t h1 = f.h1;
     const bool selfconj = (kx == 0) || (((n1 & 1) == 0) && kx == n1 / 2);
     if (!selfconj) {
-        cplx<T> g = gauss_pair<T>(static_cast<uint64_t>(idx), f.seed, f.stream);
+        const cplx<T> g = gauss_at<T>(kx + h1 * (ky + (int64_t)n2 * (kz >> 1)), kz & 1, f.seed, f.stream);
         return {g.x * rs2, g.y * rs2};
     }
     const int py = ky ? n2 - ky : 0, pz = kz ? n3 - kz : 0;
     const int64_t ls = ky + (int64_t)n2 * kz, lp = py + (int64_t)n2 * pz;
-    if (ls == lp) {
-        cplx<T> g = gauss_pair<T>(static_cast<uint64_t>(idx), f.seed, f.stream);
-        return {g.x, T(0)};
-    }
-    const int64_t canon = ls < lp ? ls : lp;
-    cplx<T> g = gauss_pair<T>(static_cast<uint64_t>(kx + (int64_t)f.h1 * canon), f.seed, f.stream);
+    const int cy = ls <= lp ? ky : py, cz = ls <= lp ? kz : pz;
+    const cplx<T> g = gauss_at<T>(kx + h1 * (cy + (int64_t)n2 * (cz >> 1)), cz & 1, f.seed, f.stream);
+    if (ls == lp) return {g.x, T(0)};
     return {g.x * rs2, (ls < lp ? g.y : -g.y) * rs2};
+}
+
+// Two neighbouring points (kx, ky, kz0) and (kx, ky, kz0 + 1), kz0 even, off the self-conjugate
+// planes: one Philox call.
+template <typename T>
+__device__ __forceinline__ void noise_h_pair(const FieldH& f, int kx, int ky, int kz0, cplx<T>& w0, cplx<T>& w1) {
+    const T rs2 = T(0.70710678118654752440084436210484904);
+    uint32_t o[4];
+    philox_at(kx + (int64_t)f.h1 * (ky + (int64_t)f.n[1] * (kz0 >> 1)), f.seed, f.stream, o);
+    const cplx<T> g0 = box_muller<T>(o[0], o[1]), g1 = box_muller<T>(o[2], o[3]);
+    w0 = {g0.x * rs2, g0.y * rs2};
+    w1 = {g1.x * rs2, g1.y * rs2};
 }
 
 template <typename T>
 __device__ __forceinline__ cplx<T> generate_h(const FieldH& f, int kx, int ky, int kz, int64_t idx) {
     cplx<T> w;
     if (f.src_mode == SRC_HOST_NOISE) w = reinterpret_cast<const cplx<T>*>(f.W)[idx];
-    else w = noise_h<T>(f, kx, ky, kz, idx);
+    else w = noise_h<T>(f, kx, ky, kz);
     const T a = amplitude_h<T>(f, kx, ky, kz);
     return {a * w.x, a * w.y};
 }
@@ -95,7 +107,30 @@ __global__ __launch_bounds__(kBlock) void k_h_strided(const FieldH* __restrict__
         return ws[idx];
     };
     auto gst = [&](const Ctx& c, int n, cplx<T> v) { ws[c.base + n * S] = v; };
-    pencil_fft<T, LAYOUT_PF, GEN, false>(pl, pool, lds, npen, prep, gld, gst);
+    // stage-in of the generating pass: along z, neighbouring kz share one Philox call
+    auto gin = [&](const Ctx& c, int b, int tpp, auto&& put) {
+        const int N = pl.N;
+        const bool paired = gen && axis == 2 && f.src_mode == SRC_DEVICE_NOISE && c.kx != 0 &&
+                            !(((f.n[0] & 1) == 0) && c.kx == f.n[0] / 2);
+        if (paired) {
+#pragma unroll 1
+            for (int q = b; 2 * q < N; q += tpp) {
+                const int kz0 = 2 * q;
+                cplx<T> w0, w1;
+                noise_h_pair<T>(f, c.kx, c.k2, kz0, w0, w1);
+                const T a0 = amplitude_h<T>(f, c.kx, c.k2, kz0);
+                put(kz0, cplx<T>{a0 * w0.x, a0 * w0.y});
+                if (kz0 + 1 < N) {
+                    const T a1 = amplitude_h<T>(f, c.kx, c.k2, kz0 + 1);
+                    put(kz0 + 1, cplx<T>{a1 * w1.x, a1 * w1.y});
+                }
+            }
+        } else {
+#pragma unroll 1
+            for (int n = b; n < N; n += tpp) put(n, gld(c, n));
+        }
+    };
+    pencil_fft<T, LAYOUT_PF, GEN, false>(pl, pool, lds, npen, prep, gld, gst, gin);
 }
 
 // x pass: half-length complex FFT of Z_k = (X_k + X*_{M-k}) + i e^{2 pi i k/n1} (X_k - X*_{M-k})
@@ -158,10 +193,11 @@ __global__ __launch_bounds__(kBlock) void k_h_xpass(const FieldH* __restrict__ f
         if (!f.blend) {
             const T r0 = apply_margi<T>(z.x, f.margi, mu, sd), r1 = apply_margi<T>(z.y, f.margi, mu, sd);
             if (mom) {
-                nadd(ms, mc, (double)r0);
-                nadd(ms2, mc2, (double)r0 * (double)r0);
-                nadd(ms, mc, (double)r1);
-                nadd(ms2, mc2, (double)r1 * (double)r1);
+                // a thread sums at most a few dozen values in plain FP64; the block, tile and
+                // field combinations are compensated
+                const double d0 = r0, d1 = r1;
+                ms += d0 + d1;
+                ms2 += d0 * d0 + d1 * d1;
             }
             if constexpr (sizeof(T) == 4) *reinterpret_cast<float2*>(o + 2 * k) = make_float2(r0, r1);
             else *reinterpret_cast<double2*>(o + 2 * k) = make_double2(r0, r1);
@@ -181,8 +217,9 @@ __global__ __launch_bounds__(kBlock) void k_h_xpass(const FieldH* __restrict__ f
             if (last) {
                 val = apply_margi<T>(val, f.margi, mu, sd);
                 if (mom) {
-                    nadd(ms, mc, (double)val);
-                    nadd(ms2, mc2, (double)val * (double)val);
+                    const double d = val;
+                    ms += d;
+                    ms2 += d * d;
                 }
             }
             o[i] = val;
