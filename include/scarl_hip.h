@@ -43,6 +43,10 @@ int scarl_hip_timing_report(char* buf, int32_t len);
 int scarl_hip_timing_reset(void);
 /* the library's HIP stream (hipStream_t), for callers that order their own work after it */
 int scarl_hip_get_stream(void** stream);
+/* plan tuning for batches created afterwards (no counterpart in the reference; the autotuner's
+ * knob).  SCARL_TUNE_FP32_ELEMS: complex elements a thread holds per FFT stage, 16 or 32. */
+enum { SCARL_TUNE_FP32_ELEMS = 1 };
+int scarl_hip_set_tuning(int32_t key, int32_t value);
 
 /* ---- fftw_ScaRL drop-in ------------------------------------------------------------------ */
 /* gen_std_gauss_FFT (SRC_F/fftw_ScaRL.f90:261-418): REDFT01 x REDFT01 x REDFT01 of

@@ -56,6 +56,7 @@ SIGNATURES = {
     "scarl_hip_timing_report": [C.c_char_p, _i32],
     "scarl_hip_timing_reset": [],
     "scarl_hip_get_stream": [C.POINTER(C.c_void_p)],
+    "scarl_hip_set_tuning": [_i32, _i32],
     "scarl_hip_gen_std_gauss_fft": [_vp, _vp, _vp, _vp, _i32, _i32, _i32, _vp, _i32],
     "scarl_hip_gen_std_gauss_fft_mpi": [_vp, _vp, _vp, _i32, _i32, _i32, _vp, _vp, _i32, _vp, _i32],
     "scarl_hip_gen_std_gauss_fft_dev": [_vp, _vp, _vp, _vp, _i32, _i32, _i32, _vp, _i32],

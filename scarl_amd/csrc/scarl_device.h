@@ -247,7 +247,9 @@ __device__ __forceinline__ cplx<T> w16(int t) {
     return {static_cast<T>(c[t & 15]), static_cast<T>(c[(t + 12) & 15])};
 }
 
-// 16 = 4 x 4: X_{q+4p} = sum_j w16^{jq} w4^{jp} DFT4_m(v_{j+4m})[q]
+// 16 = 4 x 4: X_{q+4p} = sum_j w16^{jq} w4^{jp} DFT4_m(v_{j+4m})[q].  In place: after the first
+// DFT4s v[j + 4q] holds A_j[q]; the second DFT4 over j runs on v[4q .. 4q+3], which leaves
+// X_{q+4p} at v[4q + p]; a final 4x4 transpose restores natural order.
 template <typename T>
 __device__ __forceinline__ void dft16(cplx<T>* v) {
 #pragma unroll
@@ -256,18 +258,16 @@ __device__ __forceinline__ void dft16(cplx<T>* v) {
     for (int j = 1; j < 4; ++j)
 #pragma unroll
         for (int q = 1; q < 4; ++q) v[j + 4 * q] = cmul(v[j + 4 * q], w16<T>(j * q));
-    cplx<T> o[16];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        cplx<T> a = v[4 * q], b = v[1 + 4 * q], c = v[2 + 4 * q], d = v[3 + 4 * q];
-        dft4(a, b, c, d);
-        o[q] = a;
-        o[q + 4] = b;
-        o[q + 8] = c;
-        o[q + 12] = d;
-    }
+    for (int q = 0; q < 4; ++q) dft4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
 #pragma unroll
-    for (int i = 0; i < 16; ++i) v[i] = o[i];
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = a + 1; b < 4; ++b) {
+            const cplx<T> t = v[4 * a + b];
+            v[4 * a + b] = v[4 * b + a];
+            v[4 * b + a] = t;
+        }
 }
 
 // exp(+2 pi i t / 32)
@@ -308,29 +308,30 @@ __device__ __forceinline__ cplx<T> w32(int t) {
     return {static_cast<T>(c[t & 31]), static_cast<T>(c[(t + 24) & 31])};
 }
 
-// 32 = 8 x 4: A_j = DFT8_m(v_{j+4m}), A_j[q] *= w32^{jq}, X_{q+8p} = DFT4_j(A_j[q])[p]
+// 32 = 8 x 4: A_j = DFT8_m(v_{j+4m}), A_j[q] *= w32^{jq}, X_{q+8p} = DFT4_j(A_j[q])[p].
+// In place through an 8-element gather per j (registers are renamed, not copied, once unrolled).
 template <typename T>
 __device__ __forceinline__ void dft32(cplx<T>* v) {
-    cplx<T> a[4][8];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
+        cplx<T> a[8];
 #pragma unroll
-        for (int m = 0; m < 8; ++m) a[j][m] = v[j + 4 * m];
-        dft8(a[j]);
+        for (int m = 0; m < 8; ++m) a[m] = v[j + 4 * m];
+        dft8(a);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) v[j + 4 * q] = (j && q) ? cmul(a[q], w32<T>(j * q)) : a[q];
     }
+    // v[j + 4q] = A_j[q] (twiddled); DFT4 over j for each q -> X_{q + 8p}
 #pragma unroll
-    for (int j = 1; j < 4; ++j)
+    for (int q = 0; q < 8; ++q) dft4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
+    // now v[4q + p] = X_{q + 8p}: transpose the 8 x 4 index map into natural order
+    cplx<T> o[32];
 #pragma unroll
-        for (int q = 1; q < 8; ++q) a[j][q] = cmul(a[j][q], w32<T>(j * q));
+    for (int q = 0; q < 8; ++q)
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-        cplx<T> p0 = a[0][q], p1 = a[1][q], p2 = a[2][q], p3 = a[3][q];
-        dft4(p0, p1, p2, p3);
-        v[q] = p0;
-        v[q + 8] = p1;
-        v[q + 16] = p2;
-        v[q + 24] = p3;
-    }
+        for (int p = 0; p < 4; ++p) o[q + 8 * p] = v[4 * q + p];
+#pragma unroll
+    for (int i = 0; i < 32; ++i) v[i] = o[i];
 }
 
 template <int R, typename T>

@@ -15,17 +15,6 @@
 
 namespace scarl {
 
-template <typename T>
-struct Ecap;
-template <>
-struct Ecap<float> {
-    static constexpr int v = kEcap32;
-};
-template <>
-struct Ecap<double> {
-    static constexpr int v = kEcap64;
-};
-
 // reference PI literal (SRC_F/constants_ScaRL.f90:5)
 constexpr double kPiRef = 3.1415926535898;
 
@@ -35,14 +24,14 @@ constexpr double kPiRef = 3.1415926535898;
 //   gin(ctx, b, tpp, put)  : (SIN only) writes the pencil's N inputs through put(n, v); thread b of
 //                            tpp threads; the default loops n = b, b + tpp, ... over gld
 // SIN: load the whole tile into LDS first (one copy of the loader).  SOUT: finish the transform
-// in LDS, then run gst from one loop.
+// in LDS, then run gst from one loop.  EC: complex elements a thread holds per stage (the
+// register budget; radices above EC are never planned for the instantiation).
 struct DefaultStageIn {};
 
-template <typename T, int LAYOUT, bool SIN, bool SOUT, class Prep, class GLoad, class GStore,
+template <typename T, int LAYOUT, bool SIN, bool SOUT, int EC, class Prep, class GLoad, class GStore,
           class GIn = DefaultStageIn>
 __device__ __forceinline__ void pencil_fft(const PlanDev& pl, const cplx<T>* __restrict__ pool, cplx<T>* lds,
                                            int npen, Prep&& prep, GLoad&& gld, GStore&& gst, GIn&& gin = GIn()) {
-    constexpr int EC = Ecap<T>::v;
     const int tid = threadIdx.x;
     const int Tt = pl.T, tpp = pl.tpp, N = pl.N, ldp = pl.ldp;
     int p, b;
@@ -125,6 +114,123 @@ __device__ __forceinline__ void pencil_fft(const PlanDev& pl, const cplx<T>* __r
             if (active) st_g(n, ld_l(n));
     }
 }
+
+// ------------------------------------------------------------------------------------------
+// Compile-time planned variant: N = R0 * R1 (two stages) or N = R0 (one stage), TPP threads per
+// pencil, 256 / TPP pencils per tile.  Strides, trip counts and index masks are constants, so the
+// register budget is set by the element storage alone.
+// ------------------------------------------------------------------------------------------
+template <int R, int N, int NS, int TPP, typename T, class Load, class Store>
+__device__ __forceinline__ void stage_fixed(const cplx<T>* __restrict__ tw, int b, bool active, bool sync, Load&& ld,
+                                            Store&& st) {
+    constexpr int nb = N / R;
+    constexpr int BMAX = (nb + TPP - 1) / TPP;
+    cplx<T> v[BMAX][R];
+#pragma unroll
+    for (int i = 0; i < BMAX; ++i) {
+        const int j = b + TPP * i;
+        if (active && (nb % TPP == 0 || j < nb)) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[i][r] = ld(j + r * nb);
+            if constexpr (NS > 1) {
+                const int k = j % NS;
+                const cplx<T>* t = tw + k * (R - 1);
+#pragma unroll
+                for (int r = 1; r < R; ++r) v[i][r] = cmul(v[i][r], t[r - 1]);
+            }
+            dft<R, T>(v[i]);
+        }
+    }
+    if (sync) __syncthreads();
+#pragma unroll
+    for (int i = 0; i < BMAX; ++i) {
+        const int j = b + TPP * i;
+        if (active && (nb % TPP == 0 || j < nb)) {
+            const int k = j % NS;
+            const int base = (j - k) * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) st(base + r * NS, v[i][r]);
+        }
+    }
+}
+
+template <typename T, int LAYOUT, bool SIN, bool SOUT, int N, int R0, int R1, int TPP, class Prep, class GLoad,
+          class GStore, class GIn = DefaultStageIn>
+__device__ __forceinline__ void pencil_fft_fixed(const PlanDev& pl, const cplx<T>* __restrict__ pool, cplx<T>* lds,
+                                                 int npen, Prep&& prep, GLoad&& gld, GStore&& gst,
+                                                 GIn&& gin = GIn()) {
+    static_assert(R0 * R1 == N, "plan must multiply to N");
+    constexpr int TT = kBlock / TPP;
+    constexpr int LDP = N + N / 16 + 1;
+    const int tid = threadIdx.x;
+    int p, b;
+    if constexpr (LAYOUT == LAYOUT_PF) {
+        p = tid % TT;
+        b = tid / TT;
+    } else {
+        b = tid % TPP;
+        p = tid / TPP;
+    }
+    const bool active = p < npen;
+    auto addr = [&](int n) -> int {
+        if constexpr (LAYOUT == LAYOUT_PF) return n * TT + p;
+        else return p * LDP + n + (n >> 4);
+    };
+    const auto ctx = prep(active ? p : 0);
+    auto ld_g = [&](int n) -> cplx<T> { return gld(ctx, n); };
+    auto st_g = [&](int n, cplx<T> v) { gst(ctx, n, v); };
+    auto ld_l = [&](int n) -> cplx<T> { return lds[addr(n)]; };
+    auto st_l = [&](int n, cplx<T> v) { lds[addr(n)] = v; };
+    if constexpr (SIN) {
+        if (active) {
+            if constexpr (std::is_same<typename std::decay<GIn>::type, DefaultStageIn>::value) {
+#pragma unroll 1
+                for (int n = b; n < N; n += TPP) st_l(n, ld_g(n));
+            } else {
+                gin(ctx, b, TPP, st_l);
+            }
+        }
+        __syncthreads();
+    }
+    if constexpr (R1 == 1) {
+        // single stage
+        if constexpr (SIN && SOUT) stage_fixed<R0, N, 1, TPP>(pool, b, active, true, ld_l, st_l);
+        else if constexpr (SIN) stage_fixed<R0, N, 1, TPP>(pool, b, active, true, ld_l, st_g);
+        else if constexpr (SOUT) stage_fixed<R0, N, 1, TPP>(pool, b, active, false, ld_g, st_l);
+        else stage_fixed<R0, N, 1, TPP>(pool, b, active, false, ld_g, st_g);
+    } else {
+        if constexpr (SIN) stage_fixed<R0, N, 1, TPP>(pool, b, active, true, ld_l, st_l);
+        else stage_fixed<R0, N, 1, TPP>(pool, b, active, false, ld_g, st_l);
+        __syncthreads();
+        const cplx<T>* tw = pool + pl.tw_off[1];
+        if constexpr (SOUT) stage_fixed<R1, N, R0, TPP>(tw, b, active, true, ld_l, st_l);
+        else stage_fixed<R1, N, R0, TPP>(tw, b, active, false, ld_l, st_g);
+    }
+    if constexpr (SOUT) {
+        __syncthreads();
+#pragma unroll 1
+        for (int n = b; n < N; n += TPP)
+            if (active) st_g(n, ld_l(n));
+    }
+}
+
+// FFT policies: the pass kernels take one of these as a template parameter.
+template <int EC>
+struct GenericFft {
+    template <typename T, int LAYOUT, bool SIN, bool SOUT, class... A>
+    __device__ __forceinline__ static void run(const PlanDev& pl, const cplx<T>* __restrict__ pool, cplx<T>* lds,
+                                               int npen, A&&... a) {
+        pencil_fft<T, LAYOUT, SIN, SOUT, EC>(pl, pool, lds, npen, static_cast<A&&>(a)...);
+    }
+};
+template <int N, int R0, int R1, int TPP>
+struct FixedFft {
+    template <typename T, int LAYOUT, bool SIN, bool SOUT, class... A>
+    __device__ __forceinline__ static void run(const PlanDev& pl, const cplx<T>* __restrict__ pool, cplx<T>* lds,
+                                               int npen, A&&... a) {
+        pencil_fft_fixed<T, LAYOUT, SIN, SOUT, N, R0, R1, TPP>(pl, pool, lds, npen, static_cast<A&&>(a)...);
+    }
+};
 
 // Neumaier two-sum accumulation
 __device__ __forceinline__ void nadd(double& s, double& c, double v) {

@@ -7,8 +7,7 @@ namespace scarl {
 
 constexpr int kBlock = 256;      // threads per workgroup for every FFT pass
 constexpr int kMaxStages = 8;
-constexpr int kEcap32 = 32;      // complex elements a thread holds per stage (FP32)
-constexpr int kEcap64 = 16;      // (FP64)
+constexpr int kEcap64 = 16;      // complex elements a thread holds per stage (FP64)
 
 enum Layout { LAYOUT_PF = 0, LAYOUT_BF = 1 };  // pencil-fastest lanes / butterfly-fastest lanes
 
@@ -25,6 +24,8 @@ struct PlanDev {
     int T;                 // pencils per workgroup
     int ldp;               // LDS pitch in complex elements (LAYOUT_BF)
     int lds_elems;         // complex elements of LDS the tile needs
+    int ecap;              // complex elements a thread holds per stage (generic kernel instantiation)
+    int spec;              // compile-time planned kernel id (SCARL_FIXED_SPECS), 0 = generic
 };
 
 enum SourceMode { SRC_WORKSPACE = 0, SRC_DEVICE_NOISE = 1, SRC_HOST_NOISE = 2 };
@@ -82,6 +83,29 @@ struct Tile {
     int field;
     int pencil0;
 };
+
+// Compile-time planned pass kernels: X(id, prec, layout, N, R0, R1, TPP); id 0 = generic runtime
+// plan.  prec: 4 = FP32, 8 = FP64; layout: LAYOUT_PF (strided y/z passes) or LAYOUT_BF (x pass);
+// a tile holds 256/TPP pencils and every tile stays within 64 KiB of LDS.
+#define SCARL_FIXED_SPECS(X)                \
+    X(1, 4, LAYOUT_PF, 512, 32, 16, 16)     \
+    X(2, 4, LAYOUT_PF, 256, 16, 16, 16)     \
+    X(3, 4, LAYOUT_PF, 128, 16, 8, 8)       \
+    X(4, 4, LAYOUT_PF, 64, 8, 8, 8)         \
+    X(5, 4, LAYOUT_PF, 96, 32, 3, 3)        \
+    X(6, 4, LAYOUT_PF, 160, 32, 5, 5)       \
+    X(7, 4, LAYOUT_PF, 176, 16, 11, 11)     \
+    X(8, 4, LAYOUT_BF, 256, 16, 16, 16)     \
+    X(9, 4, LAYOUT_BF, 128, 16, 8, 8)       \
+    X(10, 4, LAYOUT_BF, 64, 8, 8, 8)        \
+    X(11, 4, LAYOUT_BF, 96, 32, 3, 4)       \
+    X(12, 4, LAYOUT_BF, 80, 16, 5, 5)       \
+    X(13, 4, LAYOUT_BF, 88, 8, 11, 8)       \
+    X(14, 4, LAYOUT_BF, 48, 16, 3, 3)       \
+    X(15, 4, LAYOUT_BF, 32, 8, 4, 4)        \
+    X(16, 8, LAYOUT_PF, 256, 16, 16, 16)    \
+    X(17, 8, LAYOUT_BF, 256, 16, 16, 32)    \
+    X(18, 8, LAYOUT_BF, 128, 16, 8, 16)
 
 // Compensated (Neumaier) partial sums of x and x^2 for one tile.
 struct MomPartial {

@@ -25,7 +25,7 @@ __device__ __forceinline__ double mode_r_amp(const FieldR& f, int i, int j, int 
     return 1.0 * sqrt(S) * cos(2.0 * kPiRef * ph);
 }
 
-template <int AXIS>
+template <int AXIS, class Fft>
 __global__ __launch_bounds__(kBlock) void k_r_pass(const FieldR* __restrict__ fp, const PlanDev* __restrict__ plans,
                                                     const cplx<double>* __restrict__ pool) {
     HIP_DYNAMIC_SHARED(char, smem)
@@ -111,8 +111,8 @@ __global__ __launch_bounds__(kBlock) void k_r_pass(const FieldR* __restrict__ fp
         rout(c, 0, o, v.x);
         if (c.hasb) rout(c, 1, o, v.y);
     };
-    if constexpr (AXIS == 0) pencil_fft<double, LAYOUT_BF, true, false>(pl, pool, lds, npen, prep, gld, gst);
-    else pencil_fft<double, LAYOUT_PF, false, false>(pl, pool, lds, npen, prep, gld, gst);
+    if constexpr (AXIS == 0) Fft::template run<double, LAYOUT_BF, true, false>(pl, pool, lds, npen, prep, gld, gst);
+    else Fft::template run<double, LAYOUT_PF, false, false>(pl, pool, lds, npen, prep, gld, gst);
 }
 
 
@@ -232,16 +232,36 @@ __global__ __launch_bounds__(kBlock) void k_c2c_xcrop(const PlanDev* __restrict_
     auto gst = [&](const Ctx& x, int n, cplx<double> v) {
         if (n < np1) out[x.out + n] = v.x;
     };
-    pencil_fft<double, LAYOUT_BF, false, true>(pl, pool, lds, npen, prep, gld, gst);
+    pencil_fft<double, LAYOUT_BF, false, true, kEcap64>(pl, pool, lds, npen, prep, gld, gst);
 }
 
-hipError_t launch_r_pass(int axis, const FieldR* f, const PlanDev* plans, const void* pool, int nblocks,
+template <int AXIS, class F>
+static void r_go(const FieldR* f, const PlanDev* plans, const cplx<double>* p, int nblocks, size_t lds, hipStream_t s) {
+    hipLaunchKernelGGL((k_r_pass<AXIS, F>), dim3(nblocks), dim3(kBlock), lds, s, f, plans, p);
+}
+
+template <int AXIS>
+static void r_dispatch(int spec, const FieldR* f, const PlanDev* plans, const cplx<double>* p, int nblocks,
+                       size_t lds, hipStream_t s) {
+    constexpr int LAY = AXIS == 0 ? LAYOUT_BF : LAYOUT_PF;
+    switch (spec) {
+#define SCARL_CASE(ID, P, L, N, R0, R1, TPP)                                                  \
+    case ID:                                                                                  \
+        if constexpr (P == 8 && L == LAY) r_go<AXIS, FixedFft<N, R0, R1, TPP>>(f, plans, p, nblocks, lds, s); \
+        break;
+        SCARL_FIXED_SPECS(SCARL_CASE)
+#undef SCARL_CASE
+    default: r_go<AXIS, GenericFft<kEcap64>>(f, plans, p, nblocks, lds, s);
+    }
+}
+
+hipError_t launch_r_pass(int axis, int spec, const FieldR* f, const PlanDev* plans, const void* pool, int nblocks,
                          size_t lds_bytes, hipStream_t s) {
     const cplx<double>* p = static_cast<const cplx<double>*>(pool);
     if (nblocks <= 0) return hipSuccess;
-    if (axis == 0) hipLaunchKernelGGL(k_r_pass<0>, dim3(nblocks), dim3(kBlock), lds_bytes, s, f, plans, p);
-    else if (axis == 1) hipLaunchKernelGGL(k_r_pass<1>, dim3(nblocks), dim3(kBlock), lds_bytes, s, f, plans, p);
-    else hipLaunchKernelGGL(k_r_pass<2>, dim3(nblocks), dim3(kBlock), lds_bytes, s, f, plans, p);
+    if (axis == 0) r_dispatch<0>(spec, f, plans, p, nblocks, lds_bytes, s);
+    else if (axis == 1) r_dispatch<1>(spec, f, plans, p, nblocks, lds_bytes, s);
+    else r_dispatch<2>(spec, f, plans, p, nblocks, lds_bytes, s);
     return hipGetLastError();
 }
 

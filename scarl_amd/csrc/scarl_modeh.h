@@ -73,14 +73,14 @@ __device__ __forceinline__ cplx<T> generate_h(const FieldH& f, int kx, int ky, i
 
 // Strided pass along z (axis 2) or y (axis 1), in place on the half-spectrum workspace.
 // GEN: the pass may generate its input (staged through LDS).
-template <typename T, bool GEN>
+template <typename T, bool GEN, class Fft>
 __global__ __launch_bounds__(kBlock) void k_h_strided(const FieldH* __restrict__ fields,
-                                                       const Tile* __restrict__ tiles,
+                                                       const Tile* __restrict__ tiles, int tile0,
                                                        const PlanDev* __restrict__ plans,
                                                        const cplx<T>* __restrict__ pool, int axis) {
     HIP_DYNAMIC_SHARED(char, smem)
     cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem);
-    const Tile tl = tiles[blockIdx.x];
+    const Tile tl = tiles[tile0 + blockIdx.x];
     const FieldH& f = fields[tl.field];
     const PlanDev& pl = plans[axis == 2 ? f.plan_z : f.plan_y];
     const int h1 = f.h1, n2 = f.n[1], n3 = f.n[2];
@@ -130,7 +130,7 @@ __global__ __launch_bounds__(kBlock) void k_h_strided(const FieldH* __restrict__
             for (int n = b; n < N; n += tpp) put(n, gld(c, n));
         }
     };
-    pencil_fft<T, LAYOUT_PF, GEN, false>(pl, pool, lds, npen, prep, gld, gst, gin);
+    Fft::template run<T, LAYOUT_PF, GEN, false>(pl, pool, lds, npen, prep, gld, gst, gin);
 }
 
 // x pass: half-length complex FFT of Z_k = (X_k + X*_{M-k}) + i e^{2 pi i k/n1} (X_k - X*_{M-k})
@@ -138,7 +138,7 @@ __global__ __launch_bounds__(kBlock) void k_h_strided(const FieldH* __restrict__
 //   no blend: marginal, store, moments;
 //   blend:    value * sqrt(w_PoU); first owner (colour order) stores, later owners add, the last
 //             owner applies the marginal and takes the moments.
-template <typename T>
+template <typename T, class Fft, bool BLEND>
 __global__ __launch_bounds__(kBlock) void k_h_xpass(const FieldH* __restrict__ fields, const Tile* __restrict__ tiles,
                                                      int tile0, const PlanDev* __restrict__ plans,
                                                      const cplx<T>* __restrict__ pool,
@@ -169,7 +169,7 @@ __global__ __launch_bounds__(kBlock) void k_h_xpass(const FieldH* __restrict__ f
         const int row = tl.pencil0 + p;
         const int ky = row % n2, kz = row / n2;
         Ctx c = {ws + (int64_t)row * h1, out + ky * f.out_ld1 + kz * f.out_ld2, T(1), 0};
-        if (f.blend) {
+        if (BLEND) {
             c.wyz = pou_sqrt_weight<T>(ky, n2, f.nov_lo[1], f.nov_hi[1]) *
                     pou_sqrt_weight<T>(kz, n3, f.nov_lo[2], f.nov_hi[2]);
             c.myz = (in_overlap(ky, n2, f.nov_lo[1], f.nov_hi[1]) ? 2 : 0) |
@@ -190,7 +190,7 @@ __global__ __launch_bounds__(kBlock) void k_h_xpass(const FieldH* __restrict__ f
     };
     auto gst = [&](const Ctx& c, int k, cplx<T> z) {
         T* o = c.o;
-        if (!f.blend) {
+        if constexpr (!BLEND) {
             const T r0 = apply_margi<T>(z.x, f.margi, mu, sd), r1 = apply_margi<T>(z.y, f.margi, mu, sd);
             if (mom) {
                 // a thread sums at most a few dozen values in plain FP64; the block, tile and
@@ -201,8 +201,7 @@ __global__ __launch_bounds__(kBlock) void k_h_xpass(const FieldH* __restrict__ f
             }
             if constexpr (sizeof(T) == 4) *reinterpret_cast<float2*>(o + 2 * k) = make_float2(r0, r1);
             else *reinterpret_cast<double2*>(o + 2 * k) = make_double2(r0, r1);
-            return;
-        }
+        } else {
         const T wyz = c.wyz;
         const int myz = c.myz;
 #pragma unroll
@@ -224,38 +223,98 @@ __global__ __launch_bounds__(kBlock) void k_h_xpass(const FieldH* __restrict__ f
             }
             o[i] = val;
         }
+        }
     };
-    pencil_fft<T, LAYOUT_BF, false, true>(pl, pool, lds, npen, prep, gld, gst);
+    // the blend epilogue (PoU weights, owner order) is staged through LDS; the plain one stores from
+    // registers in the last stage
+    Fft::template run<T, LAYOUT_BF, false, BLEND>(pl, pool, lds, npen, prep, gld, gst);
     if (mom) {
         __syncthreads();
         block_reduce_mom(ms, mc, ms2, mc2, reinterpret_cast<double*>(smem), partials + tix);
     }
 }
 
+// Launch dispatch for one precision: spec 0 runs the generic kernel with register budget E1 or E2,
+// spec k > 0 the compile-time planned kernel k of SCARL_FIXED_SPECS (only those of precision PREC
+// and the pass's layout are instantiated).
+template <typename T, int PREC, int E1, int E2>
+struct ModeHDispatch {
+    template <class F>
+    static void strided(int axis, bool gen, const FieldH* fields, const Tile* tiles, int tile0, int ntiles,
+                        const PlanDev* plans, const cplx<T>* p, size_t lds, hipStream_t s) {
+        if (gen)
+            hipLaunchKernelGGL((k_h_strided<T, true, F>), dim3(ntiles), dim3(kBlock), lds, s, fields, tiles, tile0,
+                               plans, p, axis);
+        else
+            hipLaunchKernelGGL((k_h_strided<T, false, F>), dim3(ntiles), dim3(kBlock), lds, s, fields, tiles, tile0,
+                               plans, p, axis);
+    }
+    template <class F>
+    static void xpass(bool blend, const FieldH* fields, const Tile* tiles, int tile0, int ntiles,
+                      const PlanDev* plans, const cplx<T>* p, MomPartial* partials, size_t lds, hipStream_t s) {
+        if (blend)
+            hipLaunchKernelGGL((k_h_xpass<T, F, true>), dim3(ntiles), dim3(kBlock), lds, s, fields, tiles, tile0,
+                               plans, p, partials);
+        else
+            hipLaunchKernelGGL((k_h_xpass<T, F, false>), dim3(ntiles), dim3(kBlock), lds, s, fields, tiles, tile0,
+                               plans, p, partials);
+    }
+    static hipError_t run_strided(int axis, bool gen, int spec, int ecap, const FieldH* fields, const Tile* tiles,
+                                  int tile0, int ntiles, const PlanDev* plans, const void* pool, size_t lds,
+                                  hipStream_t s) {
+        if (ntiles <= 0) return hipSuccess;
+        const cplx<T>* p = static_cast<const cplx<T>*>(pool);
+        switch (spec) {
+#define SCARL_CASE(ID, P, L, N, R0, R1, TPP)                                                             \
+    case ID:                                                                                             \
+        if constexpr (P == PREC && L == LAYOUT_PF)                                                       \
+            strided<FixedFft<N, R0, R1, TPP>>(axis, gen, fields, tiles, tile0, ntiles, plans, p, lds, s); \
+        break;
+            SCARL_FIXED_SPECS(SCARL_CASE)
+#undef SCARL_CASE
+        default:
+            if (ecap == E2) strided<GenericFft<E2>>(axis, gen, fields, tiles, tile0, ntiles, plans, p, lds, s);
+            else strided<GenericFft<E1>>(axis, gen, fields, tiles, tile0, ntiles, plans, p, lds, s);
+        }
+        return hipGetLastError();
+    }
+    static hipError_t run_xpass(int spec, int ecap, bool blend, const FieldH* fields, const Tile* tiles, int tile0,
+                                int ntiles, const PlanDev* plans, const void* pool, MomPartial* partials, size_t lds,
+                                hipStream_t s) {
+        if (ntiles <= 0) return hipSuccess;
+        if (lds < 4 * kBlock * sizeof(double)) lds = 4 * kBlock * sizeof(double);
+        const cplx<T>* p = static_cast<const cplx<T>*>(pool);
+        switch (spec) {
+#define SCARL_CASE(ID, P, L, N, R0, R1, TPP)                                                                  \
+    case ID:                                                                                                  \
+        if constexpr (P == PREC && L == LAYOUT_BF)                                                            \
+            xpass<FixedFft<N, R0, R1, TPP>>(blend, fields, tiles, tile0, ntiles, plans, p, partials, lds, s); \
+        break;
+            SCARL_FIXED_SPECS(SCARL_CASE)
+#undef SCARL_CASE
+        default:
+            if (ecap == E2) xpass<GenericFft<E2>>(blend, fields, tiles, tile0, ntiles, plans, p, partials, lds, s);
+            else xpass<GenericFft<E1>>(blend, fields, tiles, tile0, ntiles, plans, p, partials, lds, s);
+        }
+        return hipGetLastError();
+    }
+};
+
 }  // namespace scarl
 
-// Instantiate the launch wrappers for one precision T with suffix SFX.
-#define SCARL_MODEH_WRAPPERS(T, SFX)                                                                      \
-    namespace scarl {                                                                                    \
-    hipError_t launch_h_strided_##SFX(int axis, bool gen, const FieldH* fields, const Tile* tiles, int ntiles, \
-                                      const PlanDev* plans, const void* pool, size_t lds_bytes, hipStream_t s) { \
-        if (ntiles <= 0) return hipSuccess;                                                              \
-        const cplx<T>* p = static_cast<const cplx<T>*>(pool);                                            \
-        if (gen)                                                                                         \
-            hipLaunchKernelGGL((k_h_strided<T, true>), dim3(ntiles), dim3(kBlock), lds_bytes, s, fields, tiles, \
-                               plans, p, axis);                                                          \
-        else                                                                                             \
-            hipLaunchKernelGGL((k_h_strided<T, false>), dim3(ntiles), dim3(kBlock), lds_bytes, s, fields, tiles, \
-                               plans, p, axis);                                                          \
-        return hipGetLastError();                                                                        \
-    }                                                                                                    \
-    hipError_t launch_h_xpass_##SFX(const FieldH* fields, const Tile* tiles, int tile0, int ntiles,       \
-                                    const PlanDev* plans, const void* pool, MomPartial* partials,         \
-                                    size_t lds_bytes, hipStream_t s) {                                   \
-        if (ntiles <= 0) return hipSuccess;                                                              \
-        if (lds_bytes < 4 * kBlock * sizeof(double)) lds_bytes = 4 * kBlock * sizeof(double);            \
-        hipLaunchKernelGGL((k_h_xpass<T>), dim3(ntiles), dim3(kBlock), lds_bytes, s, fields, tiles, tile0, plans, \
-                           static_cast<const cplx<T>*>(pool), partials);                                 \
-        return hipGetLastError();                                                                        \
-    }                                                                                                    \
+// Instantiate the launch wrappers for one precision T (SFX suffix, PREC bytes, register budgets).
+#define SCARL_MODEH_WRAPPERS(T, SFX, PREC, E1, E2)                                                          \
+    namespace scarl {                                                                                      \
+    hipError_t launch_h_strided_##SFX(int axis, bool gen, int spec, int ecap, const FieldH* fields,        \
+                                      const Tile* tiles, int tile0, int ntiles, const PlanDev* plans,      \
+                                      const void* pool, size_t lds, hipStream_t s) {                       \
+        return ModeHDispatch<T, PREC, E1, E2>::run_strided(axis, gen, spec, ecap, fields, tiles, tile0, ntiles, \
+                                                           plans, pool, lds, s);                           \
+    }                                                                                                      \
+    hipError_t launch_h_xpass_##SFX(int spec, int ecap, bool blend, const FieldH* fields, const Tile* tiles, \
+                                    int tile0, int ntiles, const PlanDev* plans, const void* pool,         \
+                                    MomPartial* partials, size_t lds, hipStream_t s) {                     \
+        return ModeHDispatch<T, PREC, E1, E2>::run_xpass(spec, ecap, blend, fields, tiles, tile0, ntiles, plans, \
+                                                         pool, partials, lds, s);                          \
+    }                                                                                                      \
     }

@@ -23,31 +23,35 @@
 namespace scarl {
 
 // launch wrappers (scarl_kernels.hip)
-hipError_t launch_h_strided_f32(int axis, bool gen, const FieldH* fields, const Tile* tiles, int ntiles,
-                                const PlanDev* plans, const void* pool, size_t lds_bytes, hipStream_t s);
-hipError_t launch_h_strided_f64(int axis, bool gen, const FieldH* fields, const Tile* tiles, int ntiles,
-                                const PlanDev* plans, const void* pool, size_t lds_bytes, hipStream_t s);
-hipError_t launch_h_xpass_f32(const FieldH* fields, const Tile* tiles, int tile0, int ntiles, const PlanDev* plans,
-                              const void* pool, MomPartial* partials, size_t lds_bytes, hipStream_t s);
-hipError_t launch_h_xpass_f64(const FieldH* fields, const Tile* tiles, int tile0, int ntiles, const PlanDev* plans,
-                              const void* pool, MomPartial* partials, size_t lds_bytes, hipStream_t s);
+hipError_t launch_h_strided_f32(int axis, bool gen, int spec, int ecap, const FieldH* fields, const Tile* tiles,
+                                int tile0, int ntiles, const PlanDev* plans, const void* pool, size_t lds,
+                                hipStream_t s);
+hipError_t launch_h_strided_f64(int axis, bool gen, int spec, int ecap, const FieldH* fields, const Tile* tiles,
+                                int tile0, int ntiles, const PlanDev* plans, const void* pool, size_t lds,
+                                hipStream_t s);
+hipError_t launch_h_xpass_f32(int spec, int ecap, bool blend, const FieldH* fields, const Tile* tiles, int tile0,
+                              int ntiles, const PlanDev* plans, const void* pool, MomPartial* partials, size_t lds,
+                              hipStream_t s);
+hipError_t launch_h_xpass_f64(int spec, int ecap, bool blend, const FieldH* fields, const Tile* tiles, int tile0,
+                              int ntiles, const PlanDev* plans, const void* pool, MomPartial* partials, size_t lds,
+                              hipStream_t s);
 
-static hipError_t launch_h_strided(bool fp64, int axis, bool gen, const FieldH* fields, const Tile* tiles,
-                                   int ntiles, const PlanDev* plans, const void* pool, size_t lds_bytes,
-                                   hipStream_t s) {
-    return fp64 ? launch_h_strided_f64(axis, gen, fields, tiles, ntiles, plans, pool, lds_bytes, s)
-                : launch_h_strided_f32(axis, gen, fields, tiles, ntiles, plans, pool, lds_bytes, s);
+static hipError_t launch_h_strided(bool fp64, int axis, bool gen, int spec, int ecap, const FieldH* fields,
+                                   const Tile* tiles, int tile0, int ntiles, const PlanDev* plans, const void* pool,
+                                   size_t lds, hipStream_t s) {
+    return fp64 ? launch_h_strided_f64(axis, gen, spec, ecap, fields, tiles, tile0, ntiles, plans, pool, lds, s)
+                : launch_h_strided_f32(axis, gen, spec, ecap, fields, tiles, tile0, ntiles, plans, pool, lds, s);
 }
-static hipError_t launch_h_xpass(bool fp64, const FieldH* fields, const Tile* tiles, int tile0, int ntiles,
-                                 const PlanDev* plans, const void* pool, MomPartial* partials, size_t lds_bytes,
-                                 hipStream_t s) {
-    return fp64 ? launch_h_xpass_f64(fields, tiles, tile0, ntiles, plans, pool, partials, lds_bytes, s)
-                : launch_h_xpass_f32(fields, tiles, tile0, ntiles, plans, pool, partials, lds_bytes, s);
+static hipError_t launch_h_xpass(bool fp64, int spec, int ecap, bool blend, const FieldH* fields, const Tile* tiles,
+                                 int tile0, int ntiles, const PlanDev* plans, const void* pool, MomPartial* partials,
+                                 size_t lds, hipStream_t s) {
+    return fp64 ? launch_h_xpass_f64(spec, ecap, blend, fields, tiles, tile0, ntiles, plans, pool, partials, lds, s)
+                : launch_h_xpass_f32(spec, ecap, blend, fields, tiles, tile0, ntiles, plans, pool, partials, lds, s);
 }
 hipError_t launch_mom(const FieldH* fields, int nfields, const int2* field_tiles, const MomPartial* partials,
                       MomPartial* field_mom, const double* slot_count, int nslots, double* slot_out, hipStream_t s);
 hipError_t launch_spectrum_sum(const FieldH& f, MomPartial* out, int nblocks, hipStream_t s);
-hipError_t launch_r_pass(int axis, const FieldR* f, const PlanDev* plans, const void* pool, int nblocks,
+hipError_t launch_r_pass(int axis, int spec, const FieldR* f, const PlanDev* plans, const void* pool, int nblocks,
                          size_t lds_bytes, hipStream_t s);
 hipError_t launch_marginal(double* x, int64_t n, int margi, double mu, double sd, hipStream_t s);
 hipError_t launch_pou(double* x, int n1, int n2, int n3, int3 lo, int3 hi, hipStream_t s);
@@ -59,9 +63,6 @@ hipError_t launch_build_sk(const void* params, void* sk, int64_t tot, hipStream_
 hipError_t launch_shino_place(const int* dims9, const void* sk, void* c, hipStream_t s);
 hipError_t launch_c2c_xcrop(const PlanDev* plans, int plan, int T, const void* pool, const void* c, int c1, int c2,
                             int np1, int np2, int np3, double* out, size_t lds, hipStream_t s);
-hipError_t launch_h_strided_f64(int axis, bool gen, const FieldH* fields, const Tile* tiles, int ntiles,
-                                const PlanDev* plans, const void* pool, size_t lds_bytes, hipStream_t s);
-
 // host mirror of the kernel parameter block of k_build_sk (scarl_k_r.hip)
 struct SkParamsHost {
     int s1, sz2, sz3;
@@ -118,6 +119,7 @@ struct Context {
     std::map<std::string, std::pair<void*, size_t>> scratch;  // named grow-only device buffers
     int margi = 0;
     double m_avg = 0.0, m_std = 1.0;
+    int fp32_ecap = 16;  // register budget (complex elements per thread) of new FP32 plans
 };
 
 static Context g_ctx;
@@ -220,6 +222,7 @@ struct PlanSet {
     PlanDev* d_plans = nullptr;
     void* d_pool = nullptr;
     bool fp64 = false;
+    int ecap = kEcap64;  // register budget of every plan of the set (matches a kernel instantiation)
     size_t max_lds = 0;
 
     ~PlanSet() {
@@ -263,15 +266,44 @@ struct PlanSet {
         snprintf(key, sizeof key, "%d/%d/%d", N, layout, kind);
         auto it = index.find(key);
         if (it != index.end()) return it->second;
-        const int ecap = fp64 ? kEcap64 : kEcap32;
         const int csz = fp64 ? 16 : 8;
         PlanDev pl;
         memset(&pl, 0, sizeof pl);
         pl.N = N;
+        pl.ecap = ecap;
         pl.direct_off = -1;
         pl.pre_off = -1;
         std::vector<int> rad;
-        if (N >= 2 && factor(N, ecap, rad)) {
+        // compile-time planned kernel for this length, precision and layout?
+        int spec = 0, sr0 = 0, sr1 = 0, stpp = 0;
+        const int prec = fp64 ? 8 : 4;
+#define SCARL_MATCH(ID, P, L, NN, A0, A1, TP)                                   \
+        if (spec == 0 && P == prec && L == layout && NN == N) {                 \
+            spec = ID;                                                          \
+            sr0 = A0;                                                           \
+            sr1 = A1;                                                           \
+            stpp = TP;                                                          \
+        }
+        SCARL_FIXED_SPECS(SCARL_MATCH)
+#undef SCARL_MATCH
+        pl.spec = spec;
+        if (spec) {
+            pl.nst = sr1 > 1 ? 2 : 1;
+            pl.R[0] = sr0;
+            pl.Ns[0] = 1;
+            if (sr1 > 1) {
+                pl.R[1] = sr1;
+                pl.Ns[1] = sr0;
+                pl.tw_off[1] = (int)(pool.size() / 2);
+                for (int k = 0; k < sr0; ++k)
+                    for (int r = 1; r < sr1; ++r) {
+                        long double c, sn;
+                        cis_ld((long long)k * r, (long long)sr0 * sr1, c, sn);
+                        pool_push(c, sn);
+                    }
+            }
+            pl.tpp = stpp;
+        } else if (N >= 2 && factor(N, ecap, rad)) {
             pl.nst = (int)rad.size();
             int Ns = 1;
             int tpp = 1;
@@ -310,9 +342,10 @@ struct PlanSet {
         pl.ldp = N + N / 16 + 1;
         // keep the LDS tile within 64 KiB so two tiles fit on a CU.  Every plan gets an LDS
         // tile, because passes that stage their input or output use it even for one stage.
+        // (compile-time planned kernels fix T = 256 / TPP; the spec table respects the bound)
         const size_t per = (size_t)(layout == LAYOUT_PF ? N : pl.ldp) * csz;
         const int Tmax = (int)std::max<size_t>(1, (64 * 1024) / per);
-        pl.T = std::min(pl.T, Tmax);
+        if (!spec) pl.T = std::min(pl.T, Tmax);
         pl.lds_elems = (int)(pl.T * (layout == LAYOUT_PF ? N : pl.ldp));
         if (kind == 1) {
             pl.pre_off = (int)(pool.size() / 2);
@@ -408,7 +441,10 @@ struct scarl_batch {
     std::vector<scarl::FieldHost> fields;
     scarl::PlanSet ps;
     std::vector<scarl::Tile> tz, ty, tx;     // tiles of the z, y and x passes
-    std::vector<int> x_color_begin;          // x tiles grouped by colour (blend): 9 offsets
+    struct TileClass {
+        int plan, t0, t1;                    // tiles [t0, t1) of one plan (one kernel launch)
+    };
+    std::vector<TileClass> cz, cy, cx;       // launches of each pass (x: colour-major for the blend)
     std::vector<int2> field_xtiles;          // per-field [begin, end) in tx
     scarl::FieldH* d_fields = nullptr;
     scarl::Tile *d_tz = nullptr, *d_ty = nullptr, *d_tx = nullptr;
@@ -421,7 +457,6 @@ struct scarl_batch {
     void* d_ws = nullptr;       // all half-spectrum workspaces
     void* d_out = nullptr;      // all outputs (or the global field)
     int64_t bytes = 0;
-    size_t lds_z = 0, lds_y = 0, lds_x = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
 
@@ -559,38 +594,54 @@ static int finish_amplitudes(scarl_batch* b) {
     return rc;
 }
 
+// Tile tables: every pass groups its tiles by plan (one launch per plan, each with its own kernel
+// and LDS size); the x pass groups by colour first so that blended subdomains run colour by colour.
 static void make_tiles(scarl_batch* b) {
     b->tz.clear();
     b->ty.clear();
     b->tx.clear();
+    b->cz.clear();
+    b->cy.clear();
+    b->cx.clear();
     b->field_xtiles.assign(b->fields.size(), make_int2(0, 0));
+    std::map<int, std::vector<int>> zp, yp;
     for (size_t i = 0; i < b->fields.size(); ++i) {
         const FieldH& f = b->fields[i].f;
-        const int64_t h1 = f.h1, n2 = f.n[1], n3 = f.n[2];
-        if (n3 > 1) {
-            const int T = b->ps.plans[f.plan_z].T;
-            for (int64_t p = 0; p < h1 * n2; p += T) b->tz.push_back({(int)i, (int)p});
-        }
-        {
-            const int T = b->ps.plans[f.plan_y].T;
-            for (int64_t p = 0; p < h1 * n3; p += T) b->ty.push_back({(int)i, (int)p});
-        }
+        if (f.n[2] > 1) zp[f.plan_z].push_back((int)i);
+        yp[f.plan_y].push_back((int)i);
     }
-    // x tiles, grouped by colour so the blend can run colour by colour
-    b->x_color_begin.assign(9, 0);
-    for (int c = 0; c < 8; ++c) {
-        b->x_color_begin[c] = (int)b->tx.size();
-        for (size_t i = 0; i < b->fields.size(); ++i) {
+    for (auto& kv : zp) {
+        const int t0 = (int)b->tz.size(), T = b->ps.plans[kv.first].T;
+        for (int i : kv.second) {
             const FieldH& f = b->fields[i].f;
-            if ((f.color & 7) != c) continue;
-            const int T = b->ps.plans[f.plan_x].T;
-            const int64_t rows = (int64_t)f.n[1] * f.n[2];
-            b->field_xtiles[i].x = (int)b->tx.size();
-            for (int64_t p = 0; p < rows; p += T) b->tx.push_back({(int)i, (int)p});
-            b->field_xtiles[i].y = (int)b->tx.size();
+            for (int64_t p = 0; p < (int64_t)f.h1 * f.n[1]; p += T) b->tz.push_back({i, (int)p});
+        }
+        b->cz.push_back({kv.first, t0, (int)b->tz.size()});
+    }
+    for (auto& kv : yp) {
+        const int t0 = (int)b->ty.size(), T = b->ps.plans[kv.first].T;
+        for (int i : kv.second) {
+            const FieldH& f = b->fields[i].f;
+            for (int64_t p = 0; p < (int64_t)f.h1 * f.n[2]; p += T) b->ty.push_back({i, (int)p});
+        }
+        b->cy.push_back({kv.first, t0, (int)b->ty.size()});
+    }
+    for (int c = 0; c < 8; ++c) {
+        std::map<int, std::vector<int>> xp;
+        for (size_t i = 0; i < b->fields.size(); ++i)
+            if ((b->fields[i].f.color & 7) == c) xp[b->fields[i].f.plan_x].push_back((int)i);
+        for (auto& kv : xp) {
+            const int t0 = (int)b->tx.size(), T = b->ps.plans[kv.first].T;
+            for (int i : kv.second) {
+                const FieldH& f = b->fields[i].f;
+                const int64_t rows = (int64_t)f.n[1] * f.n[2];
+                b->field_xtiles[i].x = (int)b->tx.size();
+                for (int64_t p = 0; p < rows; p += T) b->tx.push_back({i, (int)p});
+                b->field_xtiles[i].y = (int)b->tx.size();
+            }
+            b->cx.push_back({kv.first, t0, (int)b->tx.size()});
         }
     }
-    b->x_color_begin[8] = (int)b->tx.size();
 }
 
 template <class V>
@@ -630,18 +681,7 @@ static int finalize_batch(scarl_batch* b, const std::vector<double>& slot_count)
     }
     SCARL_TRY(b->ps.upload());
     make_tiles(b);
-    b->lds_z = b->lds_y = b->lds_x = 0;
-    const size_t csz = 2 * rsz;
-    for (auto& fh : b->fields) {
-        const FieldH& f = fh.f;
-        if (f.n[2] > 1) b->lds_z = std::max(b->lds_z, (size_t)b->ps.plans[f.plan_z].lds_elems * csz);
-        b->lds_y = std::max(b->lds_y, (size_t)b->ps.plans[f.plan_y].lds_elems * csz);
-        b->lds_x = std::max(b->lds_x, (size_t)b->ps.plans[f.plan_x].lds_elems * csz);
-    }
-    b->lds_x = std::max(b->lds_x, (size_t)(4 * kBlock * sizeof(double)));
-    SCARL_TRY(check_lds(b->lds_z));
-    SCARL_TRY(check_lds(b->lds_y));
-    SCARL_TRY(check_lds(b->lds_x));
+    for (const PlanDev& pl : b->ps.plans) SCARL_TRY(check_lds((size_t)pl.lds_elems * 2 * rsz));
     std::vector<FieldH> hf;
     for (auto& fh : b->fields) hf.push_back(fh.f);
     SCARL_TRY(upload_vec(hf, (void**)&b->d_fields));
@@ -660,20 +700,27 @@ static int finalize_batch(scarl_batch* b, const std::vector<double>& slot_count)
 static int enqueue_batch(scarl_batch* b) {
     hipStream_t s = g_ctx.stream;
     const bool f64 = b->fp64;
-    SCARL_TRY(timed("h_zpass", [&] {
-        return launch_h_strided(f64, 2, true, b->d_fields, b->d_tz, (int)b->tz.size(), b->ps.d_plans,
-                                b->ps.d_pool, b->lds_z, s);
-    }));
-    SCARL_TRY(timed("h_ypass", [&] {
-        return launch_h_strided(f64, 1, b->any_2d, b->d_fields, b->d_ty, (int)b->ty.size(), b->ps.d_plans,
-                                b->ps.d_pool, b->lds_y, s);
-    }));
-    for (int c = 0; c < 8; ++c) {
-        const int t0 = b->x_color_begin[c], t1 = b->x_color_begin[c + 1];
-        if (t1 <= t0) continue;
+    const size_t csz = f64 ? 16 : 8;
+    const int ecap = b->ps.ecap;
+    for (const auto& cl : b->cz) {
+        const PlanDev& pl = b->ps.plans[cl.plan];
+        SCARL_TRY(timed("h_zpass", [&] {
+            return launch_h_strided(f64, 2, true, pl.spec, ecap, b->d_fields, b->d_tz, cl.t0, cl.t1 - cl.t0,
+                                    b->ps.d_plans, b->ps.d_pool, (size_t)pl.lds_elems * csz, s);
+        }));
+    }
+    for (const auto& cl : b->cy) {
+        const PlanDev& pl = b->ps.plans[cl.plan];
+        SCARL_TRY(timed("h_ypass", [&] {
+            return launch_h_strided(f64, 1, b->any_2d, pl.spec, ecap, b->d_fields, b->d_ty, cl.t0, cl.t1 - cl.t0,
+                                    b->ps.d_plans, b->ps.d_pool, (size_t)pl.lds_elems * csz, s);
+        }));
+    }
+    for (const auto& cl : b->cx) {
+        const PlanDev& pl = b->ps.plans[cl.plan];
         SCARL_TRY(timed("h_xpass", [&] {
-            return launch_h_xpass(f64, b->d_fields, b->d_tx, t0, t1 - t0, b->ps.d_plans, b->ps.d_pool,
-                                  b->d_partials, b->lds_x, s);
+            return launch_h_xpass(f64, pl.spec, ecap, b->global, b->d_fields, b->d_tx, cl.t0, cl.t1 - cl.t0,
+                                  b->ps.d_plans, b->ps.d_pool, b->d_partials, (size_t)pl.lds_elems * csz, s);
         }));
     }
     if (b->nslots > 0) {
@@ -781,7 +828,7 @@ static int mode_r_run(double* d_out, const int32_t n[3], const int32_t m[3], con
             continue;
         }
         SCARL_TRY(timed(names[a], [&] {
-            return launch_r_pass(a, g_r.d_field, g_r.ps->d_plans, g_r.ps->d_pool, (int)nblocks, g_r.lds[a],
+            return launch_r_pass(a, pl.spec, g_r.d_field, g_r.ps->d_plans, g_r.ps->d_pool, (int)nblocks, g_r.lds[a],
                                  g_ctx.stream);
         }));
     }
@@ -861,6 +908,15 @@ int scarl_hip_get_stream(void** stream) {
     if (!stream) return fail(SCARL_E_INVALID, "null stream pointer");
     *stream = (void*)g_ctx.stream;
     return 0;
+}
+
+int scarl_hip_set_tuning(int32_t key, int32_t value) {
+    if (key == SCARL_TUNE_FP32_ELEMS) {
+        if (value != 16 && value != 32) return fail(SCARL_E_INVALID, "FP32 elements per thread must be 16 or 32");
+        g_ctx.fp32_ecap = value;
+        return 0;
+    }
+    return fail(SCARL_E_INVALID, "unknown tuning key %d", key);
 }
 
 int scarl_hip_set_marginal(int32_t margiFirst, double avg, double std_dev) {
@@ -1095,6 +1151,7 @@ int scarl_hip_batch_create(const scarl_field_desc* fields, int32_t count, int32_
     std::unique_ptr<scarl_batch> b(new scarl_batch());
     b->fp64 = precision == SCARL_PREC_FP64;
     b->ps.fp64 = b->fp64;
+    b->ps.ecap = b->fp64 ? kEcap64 : g_ctx.fp32_ecap;
     b->fields.resize(count);
     std::vector<double> slot_count(count);
     for (int i = 0; i < count; ++i) {
@@ -1154,6 +1211,7 @@ int scarl_hip_blend_create(const int32_t gdim[3], const int32_t nb[3], const int
     std::unique_ptr<scarl_batch> b(new scarl_batch());
     b->fp64 = precision == SCARL_PREC_FP64;
     b->ps.fp64 = b->fp64;
+    b->ps.ecap = b->fp64 ? kEcap64 : g_ctx.fp32_ecap;
     b->global = true;
     for (int a = 0; a < 3; ++a) b->gdim[a] = gdim[a];
     const int count = nb[0] * nb[1] * nb[2];
@@ -1445,12 +1503,12 @@ int scarl_hip_gen_std_gauss_shino_fft(double* h_out, const int32_t Np[3], const 
     const size_t lz = (size_t)ps.plans[pz].lds_elems * 16, ly = (size_t)ps.plans[py].lds_elems * 16,
                  lx = (size_t)ps.plans[px].lds_elems * 16;
     if (!rc) rc = timed("shino_zpass", [&] {
-        return launch_h_strided_f64(2, false, (FieldH*)d_f, (Tile*)d_tz, (int)tz.size(), ps.d_plans, ps.d_pool, lz,
-                                    g_ctx.stream);
+        return launch_h_strided_f64(2, false, ps.plans[pz].spec, kEcap64, (FieldH*)d_f, (Tile*)d_tz, 0,
+                                    (int)tz.size(), ps.d_plans, ps.d_pool, lz, g_ctx.stream);
     });
     if (!rc) rc = timed("shino_ypass", [&] {
-        return launch_h_strided_f64(1, false, (FieldH*)d_f, (Tile*)d_ty, (int)ty.size(), ps.d_plans, ps.d_pool, ly,
-                                    g_ctx.stream);
+        return launch_h_strided_f64(1, false, ps.plans[py].spec, kEcap64, (FieldH*)d_f, (Tile*)d_ty, 0,
+                                    (int)ty.size(), ps.d_plans, ps.d_pool, ly, g_ctx.stream);
     });
     if (!rc) rc = timed("shino_xpass", [&] {
         return launch_c2c_xcrop(ps.d_plans, px, ps.plans[px].T, ps.d_pool, d_c, c1, c2, Np[0], Np[1], Np[2],
