@@ -46,6 +46,87 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+COPY_PROBE_BYTES = 1 << 30
+
+
+def copy_probe(stream, nbytes=COPY_PROBE_BYTES, reps=5):
+    """Device-to-device copy of nbytes on the library stream, timed with HIP events.
+
+    Returns (GB/s counting read + write bytes, average ms per copy).  The same copy kernel
+    calibrates the FETCH_SIZE / WRITE_SIZE counters (known bytes per dispatch)."""
+    hip = C.CDLL("libamdhip64.so")
+    src, dst = C.c_void_p(), C.c_void_p()
+    for p in (src, dst):
+        if hip.hipMalloc(C.byref(p), C.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed in the copy probe")
+    if hip.hipMemsetAsync(src, 1, C.c_size_t(nbytes), stream) != 0:
+        raise RuntimeError("hipMemsetAsync failed in the copy probe")
+    ev0, ev1 = C.c_void_p(), C.c_void_p()
+    hip.hipEventCreate(C.byref(ev0))
+    hip.hipEventCreate(C.byref(ev1))
+    kind_d2d = 3
+    if hip.hipMemcpyAsync(dst, src, C.c_size_t(nbytes), kind_d2d, stream) != 0:
+        raise RuntimeError("hipMemcpyAsync failed in the copy probe")
+    hip.hipEventRecord(ev0, stream)
+    for _ in range(reps):
+        hip.hipMemcpyAsync(dst, src, C.c_size_t(nbytes), kind_d2d, stream)
+    hip.hipEventRecord(ev1, stream)
+    if hip.hipEventSynchronize(ev1) != 0:
+        raise RuntimeError("copy probe failed")
+    ms = C.c_float()
+    hip.hipEventElapsedTime(C.byref(ms), ev0, ev1)
+    avg = ms.value / reps
+    hip.hipEventDestroy(ev0)
+    hip.hipEventDestroy(ev1)
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    return 2 * nbytes / (avg * 1e-3) / 1e9, avg
+
+
+# FP32 pass kernels of the headline workload, by the template signature rocprof reports
+PASS_KERNELS = {"h_zpass": "k_h_strided<float, true", "h_ypass": "k_h_strided<float, false",
+                "h_xpass": "k_h_xpass<float"}
+
+
+def traffic_from_counters(fetch_csv, write_csv, out_json):
+    """HBM bytes per launch of each pass from two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE,
+    KiB per dispatch), each scaled by what the counter reads for the copy probe's dispatch of
+    COPY_PROBE_BYTES (largest copyBuffer dispatch in the same run)."""
+    import csv
+
+    def per_kernel(path, counter):
+        vals = {}
+        for r in csv.DictReader(open(path)):
+            if r["Counter_Name"] != counter:
+                continue
+            vals.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]))
+        return vals
+
+    res = {}
+    for counter, path in (("FETCH_SIZE", fetch_csv), ("WRITE_SIZE", write_csv)):
+        vals = per_kernel(path, counter)
+        copies = [max(v) for k, v in vals.items() if "copyBuffer" in k]
+        if not copies:
+            raise SystemExit(f"no copy kernel in {path}")
+        factor = COPY_PROBE_BYTES / (max(copies) * 1024.0)
+        res[counter] = {"factor": factor}
+        for name, pat in PASS_KERNELS.items():
+            ks = [v for k, v in vals.items() if pat in k]
+            if ks:
+                allv = [x for v in ks for x in v]
+                res[counter][name] = sum(allv) / len(allv) * 1024.0 * factor
+    out = {"method": "rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (KiB per dispatch), each scaled by "
+                     "COPY_PROBE_BYTES / counter of the copy probe dispatch",
+           "fetch_factor": res["FETCH_SIZE"]["factor"], "write_factor": res["WRITE_SIZE"]["factor"]}
+    for name in PASS_KERNELS:
+        f, w = res["FETCH_SIZE"].get(name), res["WRITE_SIZE"].get(name)
+        if f is not None and w is not None:
+            out[name] = {"read_bytes": f, "write_bytes": w, "bytes": f + w}
+    with open(out_json, "w") as fh:
+        json.dump(out, fh, indent=1)
+    print(json.dumps(out, indent=1))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -54,7 +135,13 @@ def main():
     ap.add_argument("--batch", type=int, default=4)
     ap.add_argument("--edge", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--probe-only", action="store_true", help="run the copy probe and the headline only")
+    ap.add_argument("--traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="write profiles/traffic.json from two rocprofv3 --pmc counter CSVs and exit")
     args = ap.parse_args()
+    if args.traffic:
+        traffic_from_counters(args.traffic[0], args.traffic[1], os.path.join(ROOT, "profiles", "traffic.json"))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
                           "n_gpus": args.gpus}))
@@ -65,6 +152,10 @@ def main():
 
     A.call("scarl_hip_init", 0)
     lib = A.lib()
+    stream = C.c_void_p()
+    A.call("scarl_hip_get_stream", C.byref(stream))
+    copy_gbs, copy_ms = copy_probe(stream)
+    log(f"bench: device copy probe {copy_gbs:.0f} GB/s")
     e = args.edge
     n = (e, e, e)
     L = (32.0, 32.0, 32.0)
@@ -107,13 +198,12 @@ def main():
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tfile):
-        try:
-            traffic = json.load(open(tfile)).get(dom)
-        except Exception:
-            traffic = None
+        t = json.load(open(tfile)).get(dom)
+        traffic = None if t is None else t["bytes"]
     roofline = {"bound": "hbm", "kernel": dom, "achieved": passes[dom]["GBps"], "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": passes[dom]["frac"], "traffic": traffic,
-                "bytes_per_launch": passes[dom]["alg_bytes"], "avg_ms": passes[dom]["avg_ms"]}
+                "bytes_per_launch": passes[dom]["alg_bytes"], "avg_ms": passes[dom]["avg_ms"],
+                "frac_of_copy_probe": passes[dom]["GBps"] / copy_gbs}
     step_bytes = sum(pb.values()) * B
     step_frac = step_bytes / dt / 1e9 / HBM_PEAK_GBS
     log(f"bench: passes {json.dumps(passes)}")
@@ -122,6 +212,9 @@ def main():
     f32_field0 = batch.output(0) if not args.no_cpu else None
     moments = batch.moments()
     batch.close()
+    if args.probe_only:
+        print(json.dumps({"copy_probe_GBps": copy_gbs, "value": value}), flush=True)
+        return
 
     # ---- plain FP64, one field per launch ------------------------------------------------
     log("bench: plain fp64, one field per launch")
@@ -196,6 +289,8 @@ def main():
                    "fields_per_step": B, "edge": e, "precision": "fp32 butterflies, fp64-rounded-once twiddles"},
         "roofline": roofline,
         "step_roofline_frac": round(step_frac, 4),
+        "copy_probe": {"GBps": round(copy_gbs, 1), "bytes": COPY_PROBE_BYTES, "avg_ms": round(copy_ms, 4),
+                       "step_frac_of_copy": round(step_bytes / dt / 1e9 / copy_gbs, 4)},
         "alg_bytes_per_point": sum(pb.values()) / e ** 3,
         "passes": passes,
         "plain_fp64": {"value": round(plain_value, 3), "unit": "Gpoints/s", "ms_per_field": round(dtp * 1e3, 4),

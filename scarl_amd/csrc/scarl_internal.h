@@ -45,11 +45,15 @@ struct FieldH {
     double amp0;           // amplitude prefactor (FP64, includes the variance normalisation)
     uint64_t seed;
     uint32_t stream;
+    int wp;                // workspace row length in complex elements: n1/2 (half spectrum) or n1
+    int nyq;               // 1: the kx = n1/2 column is a separate (n2, n3) plane after the rows
     int pad0;
-    const void* W;         // host-exported noise (device copy), complex, half-spectrum layout
-    void* ws;              // complex half-spectrum workspace
+    const void* W;         // host-exported noise (device copy), complex, dense (h1, n2, n3) layout
+    void* ws;              // complex workspace: rows of wp elements (planes wz apart), then the dense
+                           // (n2, n3) Nyquist plane (nyq)
     void* out;             // output base (real)
     int64_t out_ld1, out_ld2;  // output strides in elements along y and z
+    int64_t wz;            // workspace plane pitch (elements between kz planes of the rows, >= wp n2)
     // blend (partition of unity over overlapping subdomains, fused into the x pass)
     int color;             // bit d = parity of the subdomain position along axis d
     int blend;             // 0: plain store; 1: PoU-weighted first/middle/last accumulation
@@ -79,9 +83,13 @@ struct FieldR {
     double* out;           // (n1, n2, n3)
 };
 
+// Pencils of a strided pass: regular tiles cover P = kx + wp q (kx < wp) over the workspace rows,
+// Nyquist tiles cover q of the separate kx = n1/2 plane (q = ky for the z pass, kz for the y pass).
 struct Tile {
     int field;
     int pencil0;
+    int nyq;
+    int pad;
 };
 
 // Compile-time planned pass kernels: X(id, prec, layout, N, R0, R1, TPP); id 0 = generic runtime

@@ -63,9 +63,10 @@ __device__ __forceinline__ void noise_h_pair(const FieldH& f, int kx, int ky, in
 }
 
 template <typename T>
-__device__ __forceinline__ cplx<T> generate_h(const FieldH& f, int kx, int ky, int kz, int64_t idx) {
+__device__ __forceinline__ cplx<T> generate_h(const FieldH& f, int kx, int ky, int kz) {
     cplx<T> w;
-    if (f.src_mode == SRC_HOST_NOISE) w = reinterpret_cast<const cplx<T>*>(f.W)[idx];
+    if (f.src_mode == SRC_HOST_NOISE)
+        w = reinterpret_cast<const cplx<T>*>(f.W)[kx + (int64_t)f.h1 * (ky + (int64_t)f.n[1] * kz)];
     else w = noise_h<T>(f, kx, ky, kz);
     const T a = amplitude_h<T>(f, kx, ky, kz);
     return {a * w.x, a * w.y};
@@ -83,11 +84,13 @@ __global__ __launch_bounds__(kBlock) void k_h_strided(const FieldH* __restrict__
     const Tile tl = tiles[tile0 + blockIdx.x];
     const FieldH& f = fields[tl.field];
     const PlanDev& pl = plans[axis == 2 ? f.plan_z : f.plan_y];
-    const int h1 = f.h1, n2 = f.n[1], n3 = f.n[2];
-    const int npen_total = (axis == 2) ? h1 * n2 : h1 * n3;
-    const int npen = min(pl.T, npen_total - tl.pencil0);
-    const int64_t S = (axis == 2) ? (int64_t)h1 * n2 : (int64_t)h1;
-    cplx<T>* ws = reinterpret_cast<cplx<T>*>(f.ws);
+    const int wp = f.wp, n2 = f.n[1], n3 = f.n[2];
+    const int nother = (axis == 2) ? n2 : n3;
+    const bool nyq = tl.nyq != 0;
+    const int npen = min(pl.T, (nyq ? nother : wp * nother) - tl.pencil0);
+    const int64_t wz = f.wz;
+    const int64_t S = nyq ? ((axis == 2) ? (int64_t)n2 : 1) : ((axis == 2) ? wz : (int64_t)wp);
+    cplx<T>* ws = reinterpret_cast<cplx<T>*>(f.ws) + (nyq ? wz * n3 : 0);
     const bool gen = GEN && (f.first_axis == axis) && (f.src_mode != SRC_WORKSPACE);
     struct Ctx {
         int64_t base;
@@ -95,16 +98,16 @@ __global__ __launch_bounds__(kBlock) void k_h_strided(const FieldH* __restrict__
     };
     auto prep = [&](int p) -> Ctx {
         const int P = tl.pencil0 + p;
-        const int kx = P % h1, q = P / h1;
-        const int64_t base = (axis == 2) ? (int64_t)P : kx + (int64_t)h1 * n2 * q;
+        if (nyq) return {(axis == 2) ? (int64_t)P : (int64_t)n2 * P, wp, P};
+        const int kx = P % wp, q = P / wp;
+        const int64_t base = (axis == 2) ? (int64_t)P : kx + wz * q;
         return {base, kx, q};
     };
     auto gld = [&](const Ctx& c, int n) -> cplx<T> {
-        const int64_t idx = c.base + n * S;
         if constexpr (GEN) {
-            if (gen) return (axis == 2) ? generate_h<T>(f, c.kx, c.k2, n, idx) : generate_h<T>(f, c.kx, n, c.k2, idx);
+            if (gen) return (axis == 2) ? generate_h<T>(f, c.kx, c.k2, n) : generate_h<T>(f, c.kx, n, c.k2);
         }
-        return ws[idx];
+        return ws[c.base + n * S];
     };
     auto gst = [&](const Ctx& c, int n, cplx<T> v) { ws[c.base + n * S] = v; };
     // stage-in of the generating pass: along z, neighbouring kz share one Philox call
@@ -149,7 +152,7 @@ __global__ __launch_bounds__(kBlock) void k_h_xpass(const FieldH* __restrict__ f
     const Tile tl = tiles[tix];
     const FieldH& f = fields[tl.field];
     const PlanDev& pl = plans[f.plan_x];
-    const int n1 = f.n[0], n2 = f.n[1], n3 = f.n[2], h1 = f.h1, M = n1 / 2;
+    const int n1 = f.n[0], n2 = f.n[1], n3 = f.n[2], M = n1 / 2;
     const int64_t nrows = (int64_t)n2 * n3;
     const int npen = (int)min((int64_t)pl.T, nrows - (int64_t)tl.pencil0);
     const cplx<T>* ws = reinterpret_cast<const cplx<T>*>(f.ws);
@@ -160,7 +163,8 @@ __global__ __launch_bounds__(kBlock) void k_h_xpass(const FieldH* __restrict__ f
     const bool mom = f.mom_slot >= 0;
 
     struct Ctx {
-        const cplx<T>* X;
+        const cplx<T>* X;   // row (kx < M)
+        const cplx<T>* Xn;  // its kx = M element in the Nyquist plane
         T* o;
         T wyz;     // PoU weight of the row (blend)
         int myz;   // overlap mask bits of the row (blend)
@@ -168,7 +172,8 @@ __global__ __launch_bounds__(kBlock) void k_h_xpass(const FieldH* __restrict__ f
     auto prep = [&](int p) -> Ctx {
         const int row = tl.pencil0 + p;
         const int ky = row % n2, kz = row / n2;
-        Ctx c = {ws + (int64_t)row * h1, out + ky * f.out_ld1 + kz * f.out_ld2, T(1), 0};
+        Ctx c = {ws + (int64_t)ky * M + kz * f.wz, ws + f.wz * n3 + row, out + ky * f.out_ld1 + kz * f.out_ld2,
+                 T(1), 0};
         if (BLEND) {
             c.wyz = pou_sqrt_weight<T>(ky, n2, f.nov_lo[1], f.nov_hi[1]) *
                     pou_sqrt_weight<T>(kz, n3, f.nov_lo[2], f.nov_hi[2]);
@@ -178,7 +183,7 @@ __global__ __launch_bounds__(kBlock) void k_h_xpass(const FieldH* __restrict__ f
         return c;
     };
     auto gld = [&](const Ctx& c, int k) -> cplx<T> {
-        cplx<T> a = c.X[k], bb = c.X[M - k];
+        cplx<T> a = c.X[k], bb = k ? c.X[M - k] : *c.Xn;
         if (k == 0) {
             a.y = T(0);
             bb.y = T(0);

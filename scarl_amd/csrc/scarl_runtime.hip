@@ -500,6 +500,8 @@ static int build_field(scarl_batch* b, FieldHost& fh) {
     memset(&f, 0, sizeof f);
     for (int a = 0; a < 3; ++a) f.n[a] = d.n[a];
     f.h1 = d.n[0] / 2 + 1;
+    f.wp = d.n[0] / 2;
+    f.nyq = 1;
     const int dim = d.n[2] > 1 ? 3 : 2;
     f.first_axis = (dim == 3) ? 2 : 1;
     f.src_mode = SRC_DEVICE_NOISE;
@@ -594,6 +596,22 @@ static int finish_amplitudes(scarl_batch* b) {
     return rc;
 }
 
+// Tiles of one strided pass over a field: the nother pencils of the Nyquist plane (kind 1), the
+// regular tiles that hold a kx = 0 pencil (kind 2) and the other regular tiles of the workspace
+// rows (kind 0).  The first two carry the self-conjugate noise, whose unpaired draws make them
+// costlier in the generating pass.
+static void strided_tiles(int field, const FieldH& f, int nother, int T, int kind, std::vector<Tile>& out) {
+    if (kind == 1) {
+        if (f.nyq)
+            for (int p = 0; p < nother; p += T) out.push_back({field, p, 1, 0});
+        return;
+    }
+    for (int64_t p = 0; p < (int64_t)f.wp * nother; p += T) {
+        const bool has_kx0 = (p % f.wp == 0) || (p / f.wp != (p + std::min<int64_t>(T, (int64_t)f.wp * nother - p) - 1) / f.wp);
+        if (has_kx0 == (kind == 2)) out.push_back({field, (int)p, 0, 0});
+    }
+}
+
 // Tile tables: every pass groups its tiles by plan (one launch per plan, each with its own kernel
 // and LDS size); the x pass groups by colour first so that blended subdomains run colour by colour.
 static void make_tiles(scarl_batch* b) {
@@ -610,20 +628,19 @@ static void make_tiles(scarl_batch* b) {
         if (f.n[2] > 1) zp[f.plan_z].push_back((int)i);
         yp[f.plan_y].push_back((int)i);
     }
+    // one launch per plan; the costlier self-conjugate tiles go first so that the round-robin
+    // dispatch of workgroups spreads them evenly over the XCDs (in a row-major order every row's
+    // kx = 0 tile would fall on the same XCD whenever the tiles per row are a multiple of 8)
     for (auto& kv : zp) {
         const int t0 = (int)b->tz.size(), T = b->ps.plans[kv.first].T;
-        for (int i : kv.second) {
-            const FieldH& f = b->fields[i].f;
-            for (int64_t p = 0; p < (int64_t)f.h1 * f.n[1]; p += T) b->tz.push_back({i, (int)p});
-        }
+        for (int kind : {1, 2, 0})
+            for (int i : kv.second) strided_tiles(i, b->fields[i].f, b->fields[i].f.n[1], T, kind, b->tz);
         b->cz.push_back({kv.first, t0, (int)b->tz.size()});
     }
     for (auto& kv : yp) {
         const int t0 = (int)b->ty.size(), T = b->ps.plans[kv.first].T;
-        for (int i : kv.second) {
-            const FieldH& f = b->fields[i].f;
-            for (int64_t p = 0; p < (int64_t)f.h1 * f.n[2]; p += T) b->ty.push_back({i, (int)p});
-        }
+        for (int kind : {1, 2, 0})
+            for (int i : kv.second) strided_tiles(i, b->fields[i].f, b->fields[i].f.n[2], T, kind, b->ty);
         b->cy.push_back({kv.first, t0, (int)b->ty.size()});
     }
     for (int c = 0; c < 8; ++c) {
@@ -656,7 +673,14 @@ static int upload_vec(const V& v, void** d) {
 static int finalize_batch(scarl_batch* b, const std::vector<double>& slot_count) {
     const size_t rsz = b->fp64 ? 8 : 4;
     int64_t ws_elems = 0, out_elems = 0;
-    for (auto& fh : b->fields) ws_elems += (int64_t)fh.f.h1 * fh.f.n[1] * fh.f.n[2];
+    for (auto& fh : b->fields) {
+        // kz planes of the rows are padded by 4 KiB when a plane is a multiple of 64 KiB: power-of-two
+        // plane strides put every row a z-pass tile touches on the same HBM channels
+        FieldH& f = fh.f;
+        const int64_t plane = (int64_t)f.wp * f.n[1];
+        f.wz = plane + ((f.n[2] > 1 && (plane * 2 * rsz) % 65536 == 0) ? 4096 / (2 * (int64_t)rsz) : 0);
+        ws_elems += f.wz * f.n[2] + (int64_t)f.nyq * f.n[1] * f.n[2];
+    }
     b->any_2d = false;
     for (auto& fh : b->fields) b->any_2d = b->any_2d || fh.f.n[2] == 1;
     if (!b->global)
@@ -670,7 +694,7 @@ static int finalize_batch(scarl_batch* b, const std::vector<double>& slot_count)
     for (auto& fh : b->fields) {
         FieldH& f = fh.f;
         f.ws = static_cast<char*>(b->d_ws) + wo * 2 * rsz;
-        wo += (int64_t)f.h1 * f.n[1] * f.n[2];
+        wo += f.wz * f.n[2] + (int64_t)f.nyq * f.n[1] * f.n[2];
         if (!b->global) {
             f.out = static_cast<char*>(b->d_out) + oo * rsz;
             f.out_ld1 = f.n[0];
@@ -1485,14 +1509,19 @@ int scarl_hip_gen_std_gauss_shino_fft(double* h_out, const int32_t Np[3], const 
     memset(&f, 0, sizeof f);
     f.n[0] = c1; f.n[1] = c2; f.n[2] = c3;
     f.h1 = c1;
+    f.wp = c1;
+    f.wz = (int64_t)c1 * c2;
+    f.nyq = 0;
     f.plan_z = pz; f.plan_y = py; f.plan_x = px;
     f.first_axis = -1;
     f.src_mode = SRC_WORKSPACE;
     f.ws = d_c;
     f.mom_slot = -1;
     std::vector<Tile> tz, ty;
-    for (int64_t p = 0; p < (int64_t)c1 * c2; p += ps.plans[pz].T) tz.push_back({0, (int)p});
-    for (int64_t p = 0; p < (int64_t)c1 * c3; p += ps.plans[py].T) ty.push_back({0, (int)p});
+    for (int kind : {2, 0}) {
+        strided_tiles(0, f, c2, ps.plans[pz].T, kind, tz);
+        strided_tiles(0, f, c3, ps.plans[py].T, kind, ty);
+    }
     void *d_f = nullptr, *d_tz = nullptr, *d_ty = nullptr;
     std::vector<FieldH> hf(1, f);
     int rc = upload_vec(hf, &d_f);
