@@ -113,6 +113,7 @@ struct Context {
     bool ready = false;
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t gen_stream = nullptr;  // generating passes of a pipelined batch (default priority)
     bool timing = false;
     std::vector<TimedLaunch> pending;
     std::map<std::string, std::pair<long, double>> totals;
@@ -145,6 +146,7 @@ static int ensure_init() {
     if (g_ctx.device >= n) return fail(SCARL_E_INVALID, "device %d out of range (%d devices)", g_ctx.device, n);
     HIP_TRY(hipSetDevice(g_ctx.device));
     HIP_TRY(hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking));
+    HIP_TRY(hipStreamCreateWithFlags(&g_ctx.gen_stream, hipStreamNonBlocking));
     g_ctx.ready = true;
     return 0;
 }
@@ -444,7 +446,13 @@ struct scarl_batch {
     struct TileClass {
         int plan, t0, t1;                    // tiles [t0, t1) of one plan (one kernel launch)
     };
-    std::vector<TileClass> cz, cy, cx;       // launches of each pass (x: colour-major for the blend)
+    // A group of fields runs its z, y and x passes in order; in a pipelined batch the z pass of
+    // group g + 1 (compute-bound generation) overlaps the y and x passes of group g (HBM-bound).
+    struct Group {
+        std::vector<TileClass> cz, cy, cx;   // launches of each pass (x: colour-major for the blend)
+    };
+    std::vector<Group> groups;
+    std::vector<hipEvent_t> gev;             // fork event, then one per group (pipelined batches)
     std::vector<int2> field_xtiles;          // per-field [begin, end) in tx
     scarl::FieldH* d_fields = nullptr;
     scarl::Tile *d_tz = nullptr, *d_ty = nullptr, *d_tx = nullptr;
@@ -463,6 +471,7 @@ struct scarl_batch {
     ~scarl_batch() {
         if (gexec) hipGraphExecDestroy(gexec);
         if (graph) hipGraphDestroy(graph);
+        for (hipEvent_t e : gev) hipEventDestroy(e);
         for (auto& f : fields)
             if (f.noise_buf) hipFree(f.noise_buf);
         void* ptrs[] = {d_fields, d_tz, d_ty, d_tx, d_field_xtiles, d_partials, d_field_mom, d_slot_count, d_slot_out,
@@ -612,52 +621,70 @@ static void strided_tiles(int field, const FieldH& f, int nother, int T, int kin
     }
 }
 
-// Tile tables: every pass groups its tiles by plan (one launch per plan, each with its own kernel
-// and LDS size); the x pass groups by colour first so that blended subdomains run colour by colour.
+// Minimum z-pass tiles of a pipeline group: enough workgroups to fill the GPU several times over.
+constexpr int64_t kGroupTiles = 4096;
+
+// Tile tables.  Fields are cut into consecutive groups of at least kGroupTiles z-pass tiles (a
+// blended batch is one group: its x pass runs colour by colour over every subdomain).  Within a
+// group every pass has one launch per plan, each with its own kernel and LDS size; the costlier
+// self-conjugate tiles go first so that the round-robin dispatch of workgroups spreads them evenly
+// over the XCDs (in a row-major order every row's kx = 0 tile would fall on the same XCD whenever
+// the tiles per row are a multiple of 8).
 static void make_tiles(scarl_batch* b) {
     b->tz.clear();
     b->ty.clear();
     b->tx.clear();
-    b->cz.clear();
-    b->cy.clear();
-    b->cx.clear();
+    b->groups.clear();
     b->field_xtiles.assign(b->fields.size(), make_int2(0, 0));
-    std::map<int, std::vector<int>> zp, yp;
+    std::vector<std::vector<int>> members(1);
+    int64_t acc = 0;
     for (size_t i = 0; i < b->fields.size(); ++i) {
         const FieldH& f = b->fields[i].f;
-        if (f.n[2] > 1) zp[f.plan_z].push_back((int)i);
-        yp[f.plan_y].push_back((int)i);
-    }
-    // one launch per plan; the costlier self-conjugate tiles go first so that the round-robin
-    // dispatch of workgroups spreads them evenly over the XCDs (in a row-major order every row's
-    // kx = 0 tile would fall on the same XCD whenever the tiles per row are a multiple of 8)
-    for (auto& kv : zp) {
-        const int t0 = (int)b->tz.size(), T = b->ps.plans[kv.first].T;
-        for (int kind : {1, 2, 0})
-            for (int i : kv.second) strided_tiles(i, b->fields[i].f, b->fields[i].f.n[1], T, kind, b->tz);
-        b->cz.push_back({kv.first, t0, (int)b->tz.size()});
-    }
-    for (auto& kv : yp) {
-        const int t0 = (int)b->ty.size(), T = b->ps.plans[kv.first].T;
-        for (int kind : {1, 2, 0})
-            for (int i : kv.second) strided_tiles(i, b->fields[i].f, b->fields[i].f.n[2], T, kind, b->ty);
-        b->cy.push_back({kv.first, t0, (int)b->ty.size()});
-    }
-    for (int c = 0; c < 8; ++c) {
-        std::map<int, std::vector<int>> xp;
-        for (size_t i = 0; i < b->fields.size(); ++i)
-            if ((b->fields[i].f.color & 7) == c) xp[b->fields[i].f.plan_x].push_back((int)i);
-        for (auto& kv : xp) {
-            const int t0 = (int)b->tx.size(), T = b->ps.plans[kv.first].T;
-            for (int i : kv.second) {
-                const FieldH& f = b->fields[i].f;
-                const int64_t rows = (int64_t)f.n[1] * f.n[2];
-                b->field_xtiles[i].x = (int)b->tx.size();
-                for (int64_t p = 0; p < rows; p += T) b->tx.push_back({i, (int)p});
-                b->field_xtiles[i].y = (int)b->tx.size();
-            }
-            b->cx.push_back({kv.first, t0, (int)b->tx.size()});
+        if (!b->global && acc >= kGroupTiles) {
+            members.emplace_back();
+            acc = 0;
         }
+        members.back().push_back((int)i);
+        const int pz = f.n[2] > 1 ? f.plan_z : f.plan_y;
+        acc += ((int64_t)(f.wp + 1) * (f.n[2] > 1 ? f.n[1] : f.n[2])) / b->ps.plans[pz].T + 1;
+    }
+    for (const auto& mem : members) {
+        scarl_batch::Group g;
+        std::map<int, std::vector<int>> zp, yp;
+        for (int i : mem) {
+            const FieldH& f = b->fields[i].f;
+            if (f.n[2] > 1) zp[f.plan_z].push_back(i);
+            yp[f.plan_y].push_back(i);
+        }
+        for (auto& kv : zp) {
+            const int t0 = (int)b->tz.size(), T = b->ps.plans[kv.first].T;
+            for (int kind : {1, 2, 0})
+                for (int i : kv.second) strided_tiles(i, b->fields[i].f, b->fields[i].f.n[1], T, kind, b->tz);
+            g.cz.push_back({kv.first, t0, (int)b->tz.size()});
+        }
+        for (auto& kv : yp) {
+            const int t0 = (int)b->ty.size(), T = b->ps.plans[kv.first].T;
+            for (int kind : {1, 2, 0})
+                for (int i : kv.second) strided_tiles(i, b->fields[i].f, b->fields[i].f.n[2], T, kind, b->ty);
+            g.cy.push_back({kv.first, t0, (int)b->ty.size()});
+        }
+        for (int c = 0; c < 8; ++c) {
+            std::map<int, std::vector<int>> xp;
+            for (int i : mem)
+                if ((b->fields[i].f.color & 7) == c) xp[b->fields[i].f.plan_x].push_back(i);
+            for (auto& kv : xp) {
+                const int t0 = (int)b->tx.size(), T = b->ps.plans[kv.first].T;
+                for (int i : kv.second) {
+                    const FieldH& f = b->fields[i].f;
+                    const int64_t rows = (int64_t)f.n[1] * f.n[2];
+                    b->field_xtiles[i].x = (int)b->tx.size();
+                    for (int64_t p = 0; p < rows; p += T) b->tx.push_back({i, (int)p});
+                    b->field_xtiles[i].y = (int)b->tx.size();
+                }
+                g.cx.push_back({kv.first, t0, (int)b->tx.size()});
+            }
+        }
+        b->groups.push_back(std::move(g));
     }
 }
 
@@ -705,6 +732,12 @@ static int finalize_batch(scarl_batch* b, const std::vector<double>& slot_count)
     }
     SCARL_TRY(b->ps.upload());
     make_tiles(b);
+    if (b->groups.size() > 1)
+        for (size_t g = 0; g <= b->groups.size(); ++g) {
+            hipEvent_t e;
+            HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            b->gev.push_back(e);
+        }
     for (const PlanDev& pl : b->ps.plans) SCARL_TRY(check_lds((size_t)pl.lds_elems * 2 * rsz));
     std::vector<FieldH> hf;
     for (auto& fh : b->fields) hf.push_back(fh.f);
@@ -721,32 +754,52 @@ static int finalize_batch(scarl_batch* b, const std::vector<double>& slot_count)
     return 0;
 }
 
-static int enqueue_batch(scarl_batch* b) {
-    hipStream_t s = g_ctx.stream;
+// Passes of one group: z on stream zs (the generating pass of 3D fields), then y and x on s.
+static int enqueue_group(scarl_batch* b, const scarl_batch::Group& g, hipStream_t zs, hipStream_t s,
+                         hipEvent_t z_done) {
     const bool f64 = b->fp64;
     const size_t csz = f64 ? 16 : 8;
     const int ecap = b->ps.ecap;
-    for (const auto& cl : b->cz) {
+    for (const auto& cl : g.cz) {
         const PlanDev& pl = b->ps.plans[cl.plan];
         SCARL_TRY(timed("h_zpass", [&] {
             return launch_h_strided(f64, 2, true, pl.spec, ecap, b->d_fields, b->d_tz, cl.t0, cl.t1 - cl.t0,
-                                    b->ps.d_plans, b->ps.d_pool, (size_t)pl.lds_elems * csz, s);
+                                    b->ps.d_plans, b->ps.d_pool, (size_t)pl.lds_elems * csz, zs);
         }));
     }
-    for (const auto& cl : b->cy) {
+    if (z_done) {
+        HIP_TRY(hipEventRecord(z_done, zs));
+        HIP_TRY(hipStreamWaitEvent(s, z_done, 0));
+    }
+    for (const auto& cl : g.cy) {
         const PlanDev& pl = b->ps.plans[cl.plan];
         SCARL_TRY(timed("h_ypass", [&] {
             return launch_h_strided(f64, 1, b->any_2d, pl.spec, ecap, b->d_fields, b->d_ty, cl.t0, cl.t1 - cl.t0,
                                     b->ps.d_plans, b->ps.d_pool, (size_t)pl.lds_elems * csz, s);
         }));
     }
-    for (const auto& cl : b->cx) {
+    for (const auto& cl : g.cx) {
         const PlanDev& pl = b->ps.plans[cl.plan];
         SCARL_TRY(timed("h_xpass", [&] {
             return launch_h_xpass(f64, pl.spec, ecap, b->global, b->d_fields, b->d_tx, cl.t0, cl.t1 - cl.t0,
                                   b->ps.d_plans, b->ps.d_pool, b->d_partials, (size_t)pl.lds_elems * csz, s);
         }));
     }
+    return 0;
+}
+
+// One step of a batch on the library stream.  Several groups pipeline over two streams (the z
+// passes on gen_stream, each group's y and x passes on the library stream after its z pass), except
+// with event timing on, where every launch runs alone on the library stream.
+static int enqueue_batch(scarl_batch* b) {
+    hipStream_t s = g_ctx.stream;
+    const bool pipe = b->groups.size() > 1 && !g_ctx.timing;
+    if (pipe) {
+        HIP_TRY(hipEventRecord(b->gev[0], s));
+        HIP_TRY(hipStreamWaitEvent(g_ctx.gen_stream, b->gev[0], 0));
+    }
+    for (size_t g = 0; g < b->groups.size(); ++g)
+        SCARL_TRY(enqueue_group(b, b->groups[g], pipe ? g_ctx.gen_stream : s, s, pipe ? b->gev[g + 1] : nullptr));
     if (b->nslots > 0) {
         SCARL_TRY(timed("moments", [&] {
             return launch_mom(b->d_fields, (int)b->fields.size(), b->d_field_xtiles, b->d_partials, b->d_field_mom,
@@ -889,7 +942,9 @@ int scarl_hip_finalize(void) {
     g_r.d_field = nullptr;
     memset(g_r.n, 0, sizeof g_r.n);
     hipStreamDestroy(g_ctx.stream);
+    hipStreamDestroy(g_ctx.gen_stream);
     g_ctx.stream = nullptr;
+    g_ctx.gen_stream = nullptr;
     g_ctx.ready = false;
     return 0;
 }
