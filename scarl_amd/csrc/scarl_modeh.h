@@ -116,16 +116,50 @@ __global__ __launch_bounds__(kBlock) void k_h_strided(const FieldH* __restrict__
         const bool paired = gen && axis == 2 && f.src_mode == SRC_DEVICE_NOISE && c.kx != 0 &&
                             !(((f.n[0] & 1) == 0) && c.kx == f.n[0] / 2);
         if (paired) {
+            if constexpr (sizeof(T) == 4) {
+                // amp(k) W(k) = sqrt(-ln2 amp0^2 log2(u1) P(q)) (cos 2 pi u2, sin 2 pi u2) with
+                // P = (1 + q)^(2 expo) (Matern family) or 2^(-q / (4 ln 2)) (Gaussian), q = sum_d
+                // (alpha_d f_d)^2: amplitude, the 1/sqrt2 of the complex draw and the Box-Muller
+                // radius share one square root; |f_z| = min(kz, n3 - kz).
+                const float ax = float(f.alpha[0]) * float(c.kx);
+                const float ay = float(f.alpha[1]) * float(signed_freq(c.k2, f.n[1]));
+                const bool gauss = f.model == CM_GAUSSIAN;
+                const float qxy = fmaf(ax, ax, ay * ay) + (gauss ? 0.0f : 1.0f);
+                const float az = float(f.alpha[2]);
+                const float e2 = gauss ? -0.36067376022224085f : float(2.0 * f.expo);
+                const float cc = float(-0.69314718055994530942 * f.amp0 * f.amp0);
+                const int n3 = f.n[2];
+                const int64_t ctr0 = c.kx + (int64_t)f.h1 * c.k2, cstep = (int64_t)f.h1 * f.n[1];
+                auto root = [&](int kz, uint32_t u) {
+                    const float t = az * float(min(kz, n3 - kz));
+                    const float s = fmaf(t, t, qxy);
+                    const float P = __builtin_amdgcn_exp2f(e2 * (gauss ? s : __builtin_amdgcn_logf(s)));
+                    return __builtin_amdgcn_sqrtf(cc * __builtin_amdgcn_logf(u24<float>(u)) * P);
+                };
 #pragma unroll 1
-            for (int q = b; 2 * q < N; q += tpp) {
-                const int kz0 = 2 * q;
-                cplx<T> w0, w1;
-                noise_h_pair<T>(f, c.kx, c.k2, kz0, w0, w1);
-                const T a0 = amplitude_h<T>(f, c.kx, c.k2, kz0);
-                put(kz0, cplx<T>{a0 * w0.x, a0 * w0.y});
-                if (kz0 + 1 < N) {
-                    const T a1 = amplitude_h<T>(f, c.kx, c.k2, kz0 + 1);
-                    put(kz0 + 1, cplx<T>{a1 * w1.x, a1 * w1.y});
+                for (int q = b; 2 * q < N; q += tpp) {
+                    const int kz0 = 2 * q;
+                    uint32_t o[4];
+                    philox_at(ctr0 + cstep * q, f.seed, f.stream, o);
+                    const float r0 = root(kz0, o[0]), v0 = u24<float>(o[1]);
+                    put(kz0, cplx<T>{r0 * __builtin_amdgcn_cosf(v0), r0 * __builtin_amdgcn_sinf(v0)});
+                    if (kz0 + 1 < N) {
+                        const float r1 = root(kz0 + 1, o[2]), v1 = u24<float>(o[3]);
+                        put(kz0 + 1, cplx<T>{r1 * __builtin_amdgcn_cosf(v1), r1 * __builtin_amdgcn_sinf(v1)});
+                    }
+                }
+            } else {
+#pragma unroll 1
+                for (int q = b; 2 * q < N; q += tpp) {
+                    const int kz0 = 2 * q;
+                    cplx<T> w0, w1;
+                    noise_h_pair<T>(f, c.kx, c.k2, kz0, w0, w1);
+                    const T a0 = amplitude_h<T>(f, c.kx, c.k2, kz0);
+                    put(kz0, cplx<T>{a0 * w0.x, a0 * w0.y});
+                    if (kz0 + 1 < N) {
+                        const T a1 = amplitude_h<T>(f, c.kx, c.k2, kz0 + 1);
+                        put(kz0 + 1, cplx<T>{a1 * w1.x, a1 * w1.y});
+                    }
                 }
             }
         } else {
