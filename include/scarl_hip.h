@@ -97,12 +97,30 @@ int scarl_hip_apply_unity_partition(double* h_field, const int32_t Np[3], const 
                                     int32_t partition_type, const int32_t topo_pos[3], const int32_t topo_shape[3]);
 /* multiVariateTransformation (SRC_F/main_ScaRL.f90:1355-1395) */
 int scarl_hip_multivariate_transformation(double avg, double std_dev, int32_t margiFirst, double* h_field, int64_t n);
+/* normalize_field (SRC_F/main_ScaRL.f90:903-953) on one rank: the reference sums the non-duplicated
+ * box of every rank with MPI_ALLREDUCE, so topo_shape must be (1,1,1) here (otherwise
+ * SCARL_E_UNSUPPORTED).  h_field (Np, Fortran order) is normalised in place; avg_out / std_out
+ * (nullable) receive the values used. */
+int scarl_hip_normalize_field(double* h_field, const int32_t Np[3], const int32_t Np_ovlp[3], const int32_t topo_pos[3],
+                              const int32_t topo_shape[3], double* avg_out, double* std_out);
 
 /* ---- Stat_ScaRL moments ------------------------------------------------------------------ */
 /* calculate_average_and_stdVar_MPI on one rank (SRC_F/Stat_ScaRL/statistics_RF.f90:18-174):
  * h_rf is (n, nmc); FP64 compensated sums on the device. */
 int scarl_hip_moments(const double* h_rf, int64_t n, int32_t nmc, double* evnt_avg, double* evnt_std,
                       double* g_avg, double* g_std);
+/* rebuild_Sk_FFT (SRC_F/Stat_ScaRL/statistics_RF.f90:258-462): h_sk_part receives xNStep(dir)
+ * values (dir and nDim 1-based as in the reference; the taper applies when nDim = 3). */
+int scarl_hip_rebuild_sk_fft(const double* h_field, const int32_t xNStep[3], int32_t dir, int32_t nDim, double avg,
+                             double stdDev, double* h_sk_part);
+/* rebuild_corrL (SRC_F/Stat_ScaRL/statistics_RF.f90:468-486): sk_tot holds nhalf(1) + ... +
+ * nhalf(nDim) values (SkTot_Dir), corrL receives nDim values. */
+int scarl_hip_rebuild_corrl(const double* sk_tot, const int32_t nhalf[3], int32_t nDim, const double xStep[3],
+                            double* corrL);
+/* Per-axis centred covariance at lags 0..nlag-1 (no counterpart in the reference: the empirical
+ * covariance that checks a generated field against its target): h_cov[axis][lag], FP64 with
+ * compensated combination, NaN for lags beyond the extent. */
+int scarl_hip_lag_covariance(const double* h_field, const int32_t n[3], double mean, int32_t nlag, double* h_cov);
 
 /* ---- spectra / special functions --------------------------------------------------------- */
 /* K_nu(x) on the device in FP64 (the ikv scheme of SRC_F/bessel.f90:13522-13785) */
@@ -152,6 +170,10 @@ int scarl_hip_batch_output(scarl_batch* h, int32_t field, void** d_ptr, int64_t 
 int scarl_hip_batch_copy_output(scarl_batch* h, int32_t field, void* h_dst, int64_t nbytes);
 /* per slot {sum, sum of squares, count, mean, std}; nslots returned via *count */
 int scarl_hip_batch_moments(scarl_batch* h, double* h_out, int32_t max_slots, int32_t* count);
+/* after scarl_hip_batch_run of an ensemble: per-axis covariance of every field at lags 0..nlag-1,
+ * centred on the field's sample mean (from the batch moments, on the device); h_cov is
+ * [field][axis][lag].  Blended batches: SCARL_E_UNSUPPORTED (use scarl_hip_lag_covariance). */
+int scarl_hip_batch_lag_cov(scarl_batch* h, int32_t nlag, double* h_cov);
 /* amplitude prefactor and the FP64 spectrum sum used for the variance normalisation */
 int scarl_hip_batch_field_info(scarl_batch* h, int32_t field, double* amp0, double* spectrum_sum);
 /* bytes of device memory the batch holds */

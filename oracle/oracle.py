@@ -93,7 +93,7 @@ def lib():
         L.oracle_pou_axis_weight.restype = C.c_double
         L.oracle_apply_pou.argtypes = [vp, vp, vp, vp, vp]
         L.oracle_moments.argtypes = [vp, C.c_long, C.c_int, vp, vp, vp, vp, C.c_int]
-        L.oracle_lag_sums.argtypes = [vp, vp, C.c_int, vp, vp]
+        L.oracle_lag_sums.argtypes = [vp, vp, C.c_int, C.c_double, vp, vp]
         L.oracle_build_sk_shape.argtypes = [C.c_int32, vp, vp, vp, vp]
         L.oracle_build_sk_shape.restype = C.c_int
         L.oracle_build_sk.argtypes = [C.c_int32, vp, vp, vp, C.c_int32, vp]
@@ -360,12 +360,40 @@ def moments(rf, naive=False):
     return ea, es, ga.value, gs.value
 
 
-def lag_sums(x, nlag):
+def lag_sums(x, nlag, centre=0.0):
+    """Per-axis sums of (x_p - centre)(x_{p + l e_a} - centre) over the pairs inside the box and
+    the pair counts, (3, nlag) each (long double accumulation)."""
     x = np.asfortranarray(x, dtype=np.float64)
+    if x.ndim == 2:
+        x = x.reshape(x.shape + (1,), order="F")
     s = np.empty(3 * nlag)
     c = np.empty(3 * nlag)
-    lib().oracle_lag_sums(_p(x), _p(_i3(x.shape)), nlag, _p(s), _p(c))
+    lib().oracle_lag_sums(_p(x), _p(_i3(x.shape)), nlag, float(centre), _p(s), _p(c))
     return s.reshape(3, nlag), c.reshape(3, nlag)
+
+
+def lag_covariance(x, mean, nlag):
+    """Per-axis centred covariance at lags 0..nlag-1 (NaN where no pair exists)."""
+    s, c = lag_sums(x, nlag, mean)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        return np.where(c > 0, s / np.where(c > 0, c, 1), np.nan)
+
+
+def normalize_field(field, Np_ovlp=(0, 0, 0), topo_pos=(0, 0, 0), topo_shape=(1, 1, 1)):
+    """normalize_field (SRC_F/main_ScaRL.f90:903-953) on one rank: box mean over 1:maxPos divided
+    by xNTotal, then the centred sum of squares over the same box; returns (normalised copy,
+    avg, std_dev)."""
+    x = np.array(field, dtype=np.float64, order="F")
+    Np = np.array(x.shape)
+    ov = np.array(Np_ovlp)
+    ts = np.array(topo_shape)
+    mx = np.where(np.array(topo_pos) != ts - 1, Np - ov, Np)
+    xnt = float(np.prod(ts * (Np - ov) + ov))
+    box = (slice(0, mx[0]), slice(0, mx[1]), slice(0, mx[2]))
+    avg = float(np.sum(x[box], dtype=np.longdouble)) / xnt
+    x = x - avg
+    sd = float(np.sqrt(np.sum(np.square(x[box], dtype=np.longdouble)) / xnt))
+    return x / sd, avg, sd
 
 
 def build_sk_shape(corrMod, xRange, Np, sz_c):
@@ -411,24 +439,38 @@ def blend_global(fields, starts, gshape, weights_1d):
     return G
 
 
+# r_tukey = 0.1 is a default-real literal in a double precision parameter
+# (SRC_F/Stat_ScaRL/statistics_RF.f90:277): the float 0.1 widened to double
+R_TUKEY = float(np.float32(0.1))
+
+
+def stat_taper(n):
+    """The estimator's taper exactly as written (statistics_RF.f90:326-351): every ratio above r
+    takes the second cosine branch."""
+    pi = 3.1415926535898
+    r = R_TUKEY
+    t = np.ones(n)
+    for i in range(n):
+        ratio = i / (n - 1) if n > 1 else 0.0
+        if ratio < r:
+            t[i] = 0.5 * (1 + np.cos(2 * pi / r * (ratio - r / 2)))
+        elif ratio > r:
+            t[i] = 0.5 * (1 + np.cos(2 * pi / r * (ratio - 1 + r / 2)))
+    return t
+
+
 def rebuild_sk_corrl(field, xstep, avg, std):
     """Stat corrL estimator restated with numpy FFTs (SRC_F/Stat_ScaRL/statistics_RF.f90:258-486),
-    including the 'Tukey' window exactly as written (:326-351) for 3D fields."""
+    including the 'Tukey' window exactly as written (:326-351) for 3D fields.  Returns the list of
+    Sk_part arrays, one per direction."""
     x = np.asfortranarray(field, dtype=np.float64)
     nd = x.ndim
-    pi = 3.1415926535898
     Sk = []
     for d in range(nd):
         n = x.shape[d]
         v = (x - avg) / std
         if nd == 3:
-            t = np.ones(n)
-            for i in range(n):
-                ratio = i / (n - 1)
-                if ratio < 0.1:
-                    t[i] = 0.5 * (1 + np.cos(2 * pi / 0.1 * (ratio - 0.1 / 2)))
-                elif ratio > 0.1:
-                    t[i] = 0.5 * (1 + np.cos(2 * pi / 0.1 * (ratio - 1 + 0.1 / 2)))
+            t = stat_taper(n)
             shp = [1] * nd
             shp[d] = n
             v = v * t.reshape(shp)

@@ -868,9 +868,9 @@ void oracle_moments(const double *rf, long n, int nmc, double *evnt_avg, double 
     *g_std = sqrt(S2 / ((double)n * nmc) - (*g_avg) * (*g_avg));
 }
 
-/* Per-axis lag sums: cov[axis*(nlag)+l] = sum over pairs (p, p + l e_axis) inside the box
- * of x_p x_{p+l}; cnt[] = number of pairs.  Compensated long double accumulation. */
-void oracle_lag_sums(const double *x, const int n[3], int nlag, double *sums, double *cnt) {
+/* Per-axis lag sums: sums[axis*(nlag)+l] = sum over pairs (p, p + l e_axis) inside the box
+ * of (x_p - centre)(x_{p+l} - centre); cnt[] = number of pairs.  Long double accumulation. */
+void oracle_lag_sums(const double *x, const int n[3], int nlag, double centre, double *sums, double *cnt) {
     long n1 = n[0], n2 = n[1], n3 = n[2];
     long st[3] = {1, n1, n1 * n2};
     for (int a = 0; a < 3; ++a)
@@ -884,7 +884,7 @@ void oracle_lag_sums(const double *x, const int n[3], int nlag, double *sums, do
                             long idx[3] = {i, j, k};
                             if (idx[a] + l >= n[a]) continue;
                             long p = i + n1 * (j + n2 * k);
-                            s += (long double)x[p] * x[p + l * st[a]];
+                            s += ((long double)x[p] - centre) * ((long double)x[p + l * st[a]] - centre);
                             ++c;
                         }
             }

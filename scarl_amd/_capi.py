@@ -69,7 +69,11 @@ SIGNATURES = {
     "scarl_hip_r8vec_normal_01": [_i32, _pi32, _vp],
     "scarl_hip_apply_unity_partition": [_vp, _vp, _vp, _i32, _vp, _vp],
     "scarl_hip_multivariate_transformation": [_d, _d, _i32, _vp, _i64],
+    "scarl_hip_normalize_field": [_vp, _vp, _vp, _vp, _vp, _pd, _pd],
     "scarl_hip_moments": [_vp, _i64, _i32, _vp, _vp, _pd, _pd],
+    "scarl_hip_rebuild_sk_fft": [_vp, _vp, _i32, _i32, _d, _d, _vp],
+    "scarl_hip_rebuild_corrl": [_vp, _vp, _i32, _vp, _vp],
+    "scarl_hip_lag_covariance": [_vp, _vp, _d, _i32, _vp],
     "scarl_hip_kv": [_d, _vp, _i32, _vp],
     "scarl_hip_target_corr": [_i32, _d, _vp, _i32, _vp],
     "scarl_hip_batch_create": [C.POINTER(FieldDesc), _i32, _i32, C.POINTER(C.c_void_p)],
@@ -80,6 +84,7 @@ SIGNATURES = {
     "scarl_hip_batch_output": [_vp, _i32, C.POINTER(C.c_void_p), _vp],
     "scarl_hip_batch_copy_output": [_vp, _i32, _vp, _i64],
     "scarl_hip_batch_moments": [_vp, _vp, _i32, _pi32],
+    "scarl_hip_batch_lag_cov": [_vp, _i32, _vp],
     "scarl_hip_batch_field_info": [_vp, _i32, _pd, _pd],
     "scarl_hip_batch_bytes": [_vp, C.POINTER(C.c_int64)],
     "scarl_hip_batch_destroy": [_vp],

@@ -120,4 +120,28 @@ struct MomPartial {
     double s, c, s2, c2;
 };
 
+// Lag products of one array (x fastest) along one axis: for every pencil along `axis` and lag
+// l < nlag, sum_m u_m u_{m+l}, u = (x - centre) * scale (times the taper along the axis when
+// `window` is set).  circular: m + l wraps around the pencil (periodic autocorrelation, all m);
+// otherwise only pairs inside the pencil (m < n - l).
+struct LagJob {
+    const void* x;
+    int n[3];
+    int axis;
+    int64_t ld1, ld2;       // element strides along y and z
+    int nlag;
+    int circular;
+    int window;             // taper t_m of the reference's corrL estimator (r = 0.1, as written)
+    int T;                  // pencils per block
+    const double* centre;   // device scalar, or nullptr: cval
+    double cval;
+    double scale;
+    double* partial;        // [block of this job][nlag]
+    int64_t nblocks;
+};
+struct LagBlock {
+    int job;
+    int pencil0;
+};
+
 }  // namespace scarl

@@ -165,6 +165,142 @@ __global__ void k_lcg_uniform(int32_t seed0, int64_t n, double* __restrict__ r, 
 }
 
 
+// ---- lag products (per-axis covariance and the corrL estimator) -------------------------------
+// Taper of the corrL estimator as written in SRC_F/Stat_ScaRL/statistics_RF.f90:326-351: r is the
+// default-real literal 0.1 widened to double, PI the literal 3.1415926535898, and every ratio
+// above r takes the second cosine branch.
+__device__ __forceinline__ double stat_taper(int m, int n) {
+    const double r = 0.100000001490116119384765625, pi = 3.1415926535898;
+    const double ratio = n > 1 ? (double)m / (double)(n - 1) : 0.0;
+    if (ratio < r) return 0.5 * (1.0 + cos(2.0 * pi / r * (ratio - r / 2.0)));
+    if (ratio > r) return 0.5 * (1.0 + cos(2.0 * pi / r * (ratio - 1.0 + r / 2.0)));
+    return 1.0;
+}
+
+// One block: J.T pencils along J.axis staged in LDS as u[m * T + p] (FP64), then thread (p, g)
+// sums lags l = g, g + G, ... of pencil p in plain FP64 (at most a pencil length of terms); the
+// block's sum over its pencils goes to J.partial[block][l].
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_lag_partial(const LagJob* __restrict__ jobs,
+                                                         const LagBlock* __restrict__ blocks) {
+    HIP_DYNAMIC_SHARED(double, u)
+    const LagBlock bl = blocks[blockIdx.x];
+    const LagJob& J = jobs[bl.job];
+    const int ax = J.axis, n = J.n[ax], TT = J.T, nlag = J.nlag;
+    const int oa = ax == 0 ? 1 : 0, ob = ax == 2 ? 1 : 2;  // the other axes, oa fastest
+    const int64_t st[3] = {1, J.ld1, J.ld2};
+    const int64_t npen = (int64_t)J.n[oa] * J.n[ob];
+    const int np = (int)min((int64_t)TT, npen - bl.pencil0);
+    const T* __restrict__ x = static_cast<const T*>(J.x);
+    const double c = J.centre ? *J.centre : J.cval;
+    for (int idx = threadIdx.x; idx < n * TT; idx += kBlock) {
+        int p, m;
+        if (ax == 0) {  // rows are contiguous: consecutive threads walk a row
+            p = idx / n;
+            m = idx - p * n;
+        } else {        // neighbouring pencils are neighbours in x
+            p = idx % TT;
+            m = idx / TT;
+        }
+        double v = 0.0;
+        if (p < np) {
+            const int64_t P = bl.pencil0 + p;
+            const int64_t a = P % J.n[oa], b = P / J.n[oa];
+            v = ((double)x[a * st[oa] + b * st[ob] + (int64_t)m * st[ax]] - c) * J.scale;
+            if (J.window) v *= stat_taper(m, n);
+        }
+        u[m * TT + p] = v;
+    }
+    __syncthreads();
+    double* red = u + (int64_t)n * TT;  // [l][p]
+    const int G = kBlock / TT, p = threadIdx.x % TT, g = threadIdx.x / TT;
+    for (int l = g; l < nlag; l += G) {
+        double acc = 0.0;
+        if (l < n) {
+            const int mend = n - l;
+            for (int m = 0; m < mend; ++m) acc += u[m * TT + p] * u[(m + l) * TT + p];
+            if (J.circular)
+                for (int m = mend; m < n; ++m) acc += u[m * TT + p] * u[(m + l - n) * TT + p];
+        }
+        red[l * TT + p] = acc;
+    }
+    __syncthreads();
+    const int64_t bj = bl.pencil0 / TT;
+    for (int l = threadIdx.x; l < nlag; l += kBlock) {
+        double s = 0.0;
+        for (int q = 0; q < TT; ++q) s += red[l * TT + q];
+        J.partial[bj * nlag + l] = s;
+    }
+}
+
+// out[j][l] = (Neumaier sum over the job's blocks) / (pairs per pencil * pencils); undefined lags
+// (l >= n) give NaN.  Grid (nlag_max, njobs).
+__global__ __launch_bounds__(kBlock) void k_lag_finish(const LagJob* __restrict__ jobs, int nlag_max,
+                                                        double* __restrict__ out) {
+    __shared__ double red[4 * kBlock];
+    __shared__ MomPartial res;
+    const LagJob& J = jobs[blockIdx.y];
+    const int l = blockIdx.x;
+    if (l >= J.nlag) return;
+    double s = 0, cs = 0, z = 0, cz = 0;
+    for (int64_t b = threadIdx.x; b < J.nblocks; b += kBlock) nadd(s, cs, J.partial[b * J.nlag + l]);
+    block_reduce_mom(s, cs, z, cz, red, &res);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int n = J.n[J.axis];
+        const double npen = (double)J.n[0] * J.n[1] * J.n[2] / n;
+        const double pairs = J.circular ? (double)n : (double)(n - l);
+        out[(int64_t)blockIdx.y * nlag_max + l] = (l < n) ? (res.s + res.c) / (pairs * npen) : __builtin_nan("");
+    }
+}
+
+// Compensated sums over the box [0, box) of (x - centre) and (x - centre)^2 -> partial[blockIdx.x].
+__global__ __launch_bounds__(kBlock) void k_box_sums(const double* __restrict__ x, int n1, int n2, int b1, int b2,
+                                                     int b3, double centre, MomPartial* __restrict__ partial) {
+    __shared__ double red[4 * kBlock];
+    double s = 0, c = 0, s2 = 0, c2 = 0;
+    const int64_t tot = (int64_t)b1 * b2 * b3;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < tot; i += (int64_t)gridDim.x * kBlock) {
+        const int64_t a = i % b1, r = i / b1, b = r % b2, k = r / b2;
+        const double v = x[a + (int64_t)n1 * (b + (int64_t)n2 * k)] - centre;
+        nadd(s, c, v);
+        nadd(s2, c2, v * v);
+    }
+    block_reduce_mom(s, c, s2, c2, red, partial + blockIdx.x);
+}
+
+// x <- (x - a) / b over the whole array (two steps, as SRC_F/main_ScaRL.f90:930,938)
+__global__ void k_normalize(double* __restrict__ x, int64_t n, double a, double b) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        x[i] = (x[i] - a) / b;
+}
+
+hipError_t launch_lag(bool fp64, const LagJob* jobs, int njobs, const LagBlock* blocks, int64_t nblocks,
+                      size_t lds, int nlag_max, double* out, hipStream_t s) {
+    if (nblocks > 0) {
+        if (fp64) hipLaunchKernelGGL(k_lag_partial<double>, dim3((unsigned)nblocks), dim3(kBlock), lds, s, jobs, blocks);
+        else hipLaunchKernelGGL(k_lag_partial<float>, dim3((unsigned)nblocks), dim3(kBlock), lds, s, jobs, blocks);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k_lag_finish, dim3(nlag_max, njobs), dim3(kBlock), 0, s, jobs, nlag_max, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_box_sums(const double* x, const int n[3], const int box[3], double centre, MomPartial* partial,
+                           int nblocks, hipStream_t s) {
+    hipLaunchKernelGGL(k_box_sums, dim3(nblocks), dim3(kBlock), 0, s, x, n[0], n[1], box[0], box[1], box[2], centre,
+                       partial);
+    return hipGetLastError();
+}
+
+hipError_t launch_normalize(double* x, int64_t n, double a, double b, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    const int64_t nb = (n + 255) / 256 < 8192 ? (n + 255) / 256 : 8192;
+    hipLaunchKernelGGL(k_normalize, dim3((unsigned)nb), dim3(256), 0, s, x, n, a, b);
+    return hipGetLastError();
+}
+
 hipError_t launch_mom(const FieldH* fields, int nfields, const int2* field_tiles, const MomPartial* partials,
                       MomPartial* field_mom, const double* slot_count, int nslots, double* slot_out, hipStream_t s) {
     if (nfields <= 0 || nslots <= 0) return hipSuccess;

@@ -56,6 +56,11 @@ hipError_t launch_r_pass(int axis, int spec, const FieldR* f, const PlanDev* pla
 hipError_t launch_marginal(double* x, int64_t n, int margi, double mu, double sd, hipStream_t s);
 hipError_t launch_pou(double* x, int n1, int n2, int n3, int3 lo, int3 hi, hipStream_t s);
 hipError_t launch_sums(const double* rf, int64_t n, int nmc, int nblocks, MomPartial* partial, hipStream_t s);
+hipError_t launch_lag(bool fp64, const LagJob* jobs, int njobs, const LagBlock* blocks, int64_t nblocks, size_t lds,
+                      int nlag_max, double* out, hipStream_t s);
+hipError_t launch_box_sums(const double* x, const int n[3], const int box[3], double centre, MomPartial* partial,
+                           int nblocks, hipStream_t s);
+hipError_t launch_normalize(double* x, int64_t n, double a, double b, hipStream_t s);
 hipError_t launch_lcg_uniform(int32_t seed0, int64_t n, double* r, int32_t* seeds, hipStream_t s);
 hipError_t launch_target_corr(int model, double nu, const double* rho, int n, double* out, hipStream_t s);
 hipError_t launch_kv(double nu, const double* x, int n, double* out, hipStream_t s);
@@ -1185,6 +1190,217 @@ int scarl_hip_moments(const double* h_rf, int64_t n, int32_t nmc, double* evnt_a
     const double tot = (double)n * nmc;
     *g_avg = (ts + tc) / tot;
     *g_std = sqrt((ts2 + tc2) / tot - (*g_avg) * (*g_avg));
+    return 0;
+}
+
+// ---- lag products: per-axis covariance and the corrL estimator ------------------------------
+// Runs every job in one launch (block table over jobs) and writes [job][nlag_max] to d_out.
+// Pencils per block: the largest power of two (at most 32) whose (n + nlag) T doubles fit 64 KiB.
+static int run_lag_jobs(std::vector<LagJob>& jobs, bool fp64, int nlag_max, double* d_out) {
+    size_t lds = 0;
+    int64_t tot_partial = 0;
+    std::vector<LagBlock> blocks;
+    for (size_t j = 0; j < jobs.size(); ++j) {
+        LagJob& J = jobs[j];
+        const int n = J.n[J.axis];
+        if ((size_t)(n + J.nlag) * 8 > 65536)
+            return fail(SCARL_E_UNSUPPORTED, "a pencil of %d points with %d lags exceeds the 64 KiB tile", n, J.nlag);
+        int T = 32;
+        while (T > 1 && (size_t)(n + J.nlag) * T * 8 > 65536) T >>= 1;
+        J.T = T;
+        const int64_t npen = (int64_t)J.n[0] * J.n[1] * J.n[2] / n;
+        if (npen > INT32_MAX) return fail(SCARL_E_UNSUPPORTED, "too many pencils (%lld)", (long long)npen);
+        J.nblocks = (npen + T - 1) / T;
+        for (int64_t p = 0; p < npen; p += T) blocks.push_back({(int)j, (int)p});
+        tot_partial += J.nblocks * J.nlag;
+        lds = std::max(lds, (size_t)(n + J.nlag) * T * 8);
+    }
+    void *d_part, *d_jobs, *d_blocks;
+    SCARL_TRY(scratch_get("lag_partial", (size_t)std::max<int64_t>(1, tot_partial) * 8, &d_part));
+    int64_t off = 0;
+    for (LagJob& J : jobs) {
+        J.partial = static_cast<double*>(d_part) + off;
+        off += J.nblocks * J.nlag;
+    }
+    SCARL_TRY(scratch_get("lag_jobs", jobs.size() * sizeof(LagJob), &d_jobs));
+    SCARL_TRY(scratch_get("lag_blocks", std::max<size_t>(1, blocks.size()) * sizeof(LagBlock), &d_blocks));
+    HIP_TRY(hipMemcpyAsync(d_jobs, jobs.data(), jobs.size() * sizeof(LagJob), hipMemcpyHostToDevice, g_ctx.stream));
+    if (!blocks.empty())
+        HIP_TRY(hipMemcpyAsync(d_blocks, blocks.data(), blocks.size() * sizeof(LagBlock), hipMemcpyHostToDevice,
+                               g_ctx.stream));
+    SCARL_TRY(timed("lag", [&] {
+        return launch_lag(fp64, (const LagJob*)d_jobs, (int)jobs.size(), (const LagBlock*)d_blocks,
+                          (int64_t)blocks.size(), lds, nlag_max, d_out, g_ctx.stream);
+    }));
+    HIP_TRY(hipStreamSynchronize(g_ctx.stream));  // the host job and block tables die on return
+    return 0;
+}
+
+static LagJob lag_job(const void* x, const int32_t n[3], int axis, int nlag, int circular, int window, double centre,
+                      const double* d_centre, double scale) {
+    LagJob J;
+    memset(&J, 0, sizeof J);
+    J.x = x;
+    for (int a = 0; a < 3; ++a) J.n[a] = n[a];
+    J.axis = axis;
+    J.ld1 = n[0];
+    J.ld2 = (int64_t)n[0] * n[1];
+    J.nlag = nlag;
+    J.circular = circular;
+    J.window = window;
+    J.centre = d_centre;
+    J.cval = centre;
+    J.scale = scale;
+    return J;
+}
+
+// SRC_F/Stat_ScaRL/statistics_RF.f90:258-462 rebuild_Sk_FFT: Sk_part(l) = Re IFFT(mean over the
+// pencils along dir of |FFT(u)/N|^2), u = (x - avg)/stdDev tapered along dir when nDim = 3, which is
+// the periodic autocorrelation (1/N) sum_m u_m u_{(m+l) mod N} averaged over the pencils; computed
+// directly in FP64 with compensated block combination.
+int scarl_hip_rebuild_sk_fft(const double* h_field, const int32_t xNStep[3], int32_t dir, int32_t nDim, double avg,
+                             double stdDev, double* h_sk_part) {
+    using namespace scarl;
+    SCARL_TRY(ensure_init());
+    if (!h_field || !xNStep || !h_sk_part) return fail(SCARL_E_INVALID, "null argument");
+    if (nDim < 1 || nDim > 3 || dir < 1 || dir > nDim) return fail(SCARL_E_INVALID, "bad nDim %d / dir %d", nDim, dir);
+    int32_t n[3] = {1, 1, 1};
+    for (int a = 0; a < nDim; ++a) {
+        if (xNStep[a] < 1) return fail(SCARL_E_INVALID, "xNStep(%d) = %d", a + 1, xNStep[a]);
+        n[a] = xNStep[a];
+    }
+    if (!(stdDev != 0.0)) return fail(SCARL_E_INVALID, "stdDev must be nonzero");
+    const int64_t tot = (int64_t)n[0] * n[1] * n[2];
+    const int nd = n[dir - 1];
+    void *d_x, *d_out;
+    SCARL_TRY(scratch_get("stat_field", (size_t)tot * 8, &d_x));
+    SCARL_TRY(scratch_get("stat_sk", (size_t)nd * 8, &d_out));
+    HIP_TRY(hipMemcpyAsync(d_x, h_field, (size_t)tot * 8, hipMemcpyHostToDevice, g_ctx.stream));
+    std::vector<LagJob> jobs(1, lag_job(d_x, n, dir - 1, nd, 1, nDim == 3, avg, nullptr, 1.0 / stdDev));
+    SCARL_TRY(run_lag_jobs(jobs, true, nd, (double*)d_out));
+    HIP_TRY(hipMemcpy(h_sk_part, d_out, (size_t)nd * 8, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+// SRC_F/Stat_ScaRL/statistics_RF.f90:468-486 rebuild_corrL: corrL(d) = 2 (sum(Sk_d) - (first + last)/2)
+// xStep(d) over the nhalf(d) kept lags of each direction (host arithmetic).
+int scarl_hip_rebuild_corrl(const double* sk_tot, const int32_t nhalf[3], int32_t nDim, const double xStep[3],
+                            double* corrL) {
+    if (!sk_tot || !nhalf || !xStep || !corrL) return scarl::fail(SCARL_E_INVALID, "null argument");
+    if (nDim < 1 || nDim > 3) return scarl::fail(SCARL_E_INVALID, "bad nDim %d", nDim);
+    int64_t off = 0;
+    for (int d = 0; d < nDim; ++d) {
+        if (nhalf[d] < 1) return scarl::fail(SCARL_E_INVALID, "nhalf(%d) = %d", d + 1, nhalf[d]);
+        double s = 0.0;
+        for (int l = 0; l < nhalf[d]; ++l) s += sk_tot[off + l];
+        corrL[d] = 2.0 * (s - (sk_tot[off] + sk_tot[off + nhalf[d] - 1]) / 2.0) * xStep[d];
+        off += nhalf[d];
+    }
+    return 0;
+}
+
+// Per-axis centred covariance of one field at lags 0..nlag-1: cov[a][l] = sum over the pairs
+// (p, p + l e_a) inside the box of (x_p - mean)(x_{p+l} - mean) / (number of pairs); NaN for l >= n_a.
+int scarl_hip_lag_covariance(const double* h_field, const int32_t n[3], double mean, int32_t nlag, double* h_cov) {
+    using namespace scarl;
+    SCARL_TRY(ensure_init());
+    if (!h_field || !n || !h_cov) return fail(SCARL_E_INVALID, "null argument");
+    if (nlag < 1) return fail(SCARL_E_INVALID, "nlag must be >= 1");
+    for (int a = 0; a < 3; ++a)
+        if (n[a] < 1) return fail(SCARL_E_INVALID, "n[%d] = %d", a, n[a]);
+    const int64_t tot = (int64_t)n[0] * n[1] * n[2];
+    void *d_x, *d_out;
+    SCARL_TRY(scratch_get("stat_field", (size_t)tot * 8, &d_x));
+    SCARL_TRY(scratch_get("stat_cov", (size_t)3 * nlag * 8, &d_out));
+    HIP_TRY(hipMemcpyAsync(d_x, h_field, (size_t)tot * 8, hipMemcpyHostToDevice, g_ctx.stream));
+    std::vector<LagJob> jobs;
+    for (int a = 0; a < 3; ++a) jobs.push_back(lag_job(d_x, n, a, nlag, 0, 0, mean, nullptr, 1.0));
+    SCARL_TRY(run_lag_jobs(jobs, true, nlag, (double*)d_out));
+    HIP_TRY(hipMemcpy(h_cov, d_out, (size_t)3 * nlag * 8, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+// Every field of an ensemble batch (after scarl_hip_batch_run): per-axis covariance at lags
+// 0..nlag-1 centred on the field's own sample mean (read on the device from the batch moments).
+// h_cov: [field][axis][lag].
+int scarl_hip_batch_lag_cov(scarl_batch* b, int32_t nlag, double* h_cov) {
+    using namespace scarl;
+    SCARL_TRY(ensure_init());
+    if (!b || !h_cov) return fail(SCARL_E_INVALID, "null argument");
+    if (nlag < 1) return fail(SCARL_E_INVALID, "nlag must be >= 1");
+    if (b->global)
+        return fail(SCARL_E_UNSUPPORTED, "a blended batch has one global field: use scarl_hip_lag_covariance on it");
+    std::vector<LagJob> jobs;
+    for (size_t i = 0; i < b->fields.size(); ++i) {
+        const FieldH& f = b->fields[i].f;
+        const double* mean = b->d_slot_out + 5 * f.mom_slot + 3;
+        for (int a = 0; a < 3; ++a) jobs.push_back(lag_job(f.out, f.n, a, nlag, 0, 0, 0.0, mean, 1.0));
+    }
+    void* d_out;
+    SCARL_TRY(scratch_get("batch_cov", jobs.size() * (size_t)nlag * 8, &d_out));
+    SCARL_TRY(run_lag_jobs(jobs, b->fp64, nlag, (double*)d_out));
+    HIP_TRY(hipMemcpy(h_cov, d_out, jobs.size() * (size_t)nlag * 8, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+// SRC_F/main_ScaRL.f90:903-953 normalize_field for one rank (communicator of size one): the mean
+// over the non-duplicated box 1:maxPos (maxPos = Np - Np_ovlp on sides that are not last), divided
+// by xNTotal = prod(topo_shape (Np - Np_ovlp) + Np_ovlp); then the centred sum of squares over the
+// same box; x <- (x - avg) / std_dev over the whole field.  Compensated FP64 sums.
+int scarl_hip_normalize_field(double* h_field, const int32_t Np[3], const int32_t Np_ovlp[3], const int32_t topo_pos[3],
+                              const int32_t topo_shape[3], double* avg_out, double* std_out) {
+    using namespace scarl;
+    SCARL_TRY(ensure_init());
+    if (!h_field || !Np || !Np_ovlp || !topo_pos || !topo_shape) return fail(SCARL_E_INVALID, "null argument");
+    int n[3], box[3];
+    double xnt = 1.0;
+    for (int a = 0; a < 3; ++a) {
+        if (Np[a] < 1 || Np_ovlp[a] < 0 || Np_ovlp[a] >= Np[a] || topo_shape[a] < 1 || topo_pos[a] < 0 ||
+            topo_pos[a] >= topo_shape[a])
+            return fail(SCARL_E_INVALID, "bad geometry on axis %d", a);
+        if (topo_shape[a] != 1)
+            return fail(SCARL_E_UNSUPPORTED,
+                        "normalize_field over %d ranks needs the global sums of every rank; one GPU holds one rank",
+                        topo_shape[0] * topo_shape[1] * topo_shape[2]);
+        n[a] = Np[a];
+        box[a] = (topo_pos[a] != topo_shape[a] - 1) ? Np[a] - Np_ovlp[a] : Np[a];
+        xnt *= (double)topo_shape[a] * (Np[a] - Np_ovlp[a]) + Np_ovlp[a];
+    }
+    const int64_t tot = (int64_t)n[0] * n[1] * n[2];
+    const int nblocks = (int)std::min<int64_t>(std::max<int64_t>(1, tot / (kBlock * 16)), 1024);
+    void *d_x, *d_part;
+    SCARL_TRY(scratch_get("norm_field", (size_t)tot * 8, &d_x));
+    SCARL_TRY(scratch_get("norm_part", (size_t)nblocks * sizeof(MomPartial), &d_part));
+    HIP_TRY(hipMemcpyAsync(d_x, h_field, (size_t)tot * 8, hipMemcpyHostToDevice, g_ctx.stream));
+    std::vector<MomPartial> h(nblocks);
+    auto box_sum = [&](double centre, bool squares, double* out) -> int {
+        SCARL_TRY(timed("box_sums", [&] {
+            return launch_box_sums((const double*)d_x, n, box, centre, (MomPartial*)d_part, nblocks, g_ctx.stream);
+        }));
+        HIP_TRY(hipMemcpyAsync(h.data(), d_part, h.size() * sizeof(MomPartial), hipMemcpyDeviceToHost, g_ctx.stream));
+        HIP_TRY(hipStreamSynchronize(g_ctx.stream));
+        double s = 0.0, c = 0.0;
+        for (const MomPartial& m : h) {
+            const double v = squares ? m.s2 : m.s;
+            const double t = s + v;
+            c += (fabs(s) >= fabs(v)) ? (s - t) + v : (v - t) + s;
+            s = t;
+            c += squares ? m.c2 : m.c;
+        }
+        *out = s + c;
+        return 0;
+    };
+    double S, S2;
+    SCARL_TRY(box_sum(0.0, false, &S));
+    const double avg = S / xnt;
+    SCARL_TRY(box_sum(avg, true, &S2));
+    const double sd = sqrt(S2 / xnt);
+    if (!(sd > 0.0)) return fail(SCARL_E_INVALID, "the field is constant over the box: std_dev = 0");
+    SCARL_TRY(timed("normalize", [&] { return launch_normalize((double*)d_x, tot, avg, sd, g_ctx.stream); }));
+    HIP_TRY(hipMemcpyAsync(h_field, d_x, (size_t)tot * 8, hipMemcpyDeviceToHost, g_ctx.stream));
+    HIP_TRY(hipStreamSynchronize(g_ctx.stream));
+    if (avg_out) *avg_out = avg;
+    if (std_out) *std_out = sd;
     return 0;
 }
 

@@ -133,6 +133,13 @@ class Batch:
         A.call("scarl_hip_batch_moments", self._h, A.ptr(out), out.shape[0], C.byref(cnt))
         return out[:cnt.value]
 
+    def lag_covariance(self, nlag):
+        """Ensemble batches, after run(): (nfields, 3, nlag) per-axis covariance of every field at
+        lags 0..nlag-1, centred on the field's own sample mean (FP64, compensated)."""
+        out = np.empty((len(self.shapes), 3, int(nlag)))
+        A.call("scarl_hip_batch_lag_cov", self._h, int(nlag), A.ptr(out))
+        return out
+
     def field_info(self, i):
         a = C.c_double()
         s = C.c_double()

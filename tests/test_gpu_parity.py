@@ -363,3 +363,92 @@ def test_shino_fft(hip):
     # too coarse a mesh: the reference would write outside c_in; here it is an error
     with pytest.raises(ScarlError):
         spectra_ScaRL.gen_std_gauss_Shino_FFT(out, Np, (19.0, 17.0, 15.0), (4.0, 4.0, 3.0), (5, 5, 5), 1, 3, 0)
+
+
+# ---- Stat_ScaRL: corrL estimator, per-axis covariance; normalize_field ------------------------
+def test_rebuild_sk_fft_matches_oracle(hip):
+    from scarl_amd import statistics
+    rng = np.random.default_rng(21)
+    x3 = np.asfortranarray(rng.standard_normal((48, 40, 36)) * 1.5 + 0.25)
+    ref = O.rebuild_sk_corrl(x3, 1.0, 0.25, 1.5)
+    for d in (1, 2, 3):
+        sk = statistics.rebuild_Sk_FFT(x3, x3.shape, d, 3, 0.25, 1.5)
+        np.testing.assert_allclose(sk, ref[d - 1], rtol=0, atol=1e-12 * np.abs(ref[d - 1]).max())
+    x2 = np.asfortranarray(rng.standard_normal((64, 50)))
+    ref2 = O.rebuild_sk_corrl(x2, 1.0, 0.0, 1.0)
+    for d in (1, 2):
+        sk = statistics.rebuild_Sk_FFT(x2, x2.shape, d, 2, 0.0, 1.0)
+        np.testing.assert_allclose(sk, ref2[d - 1], rtol=0, atol=1e-12 * np.abs(ref2[d - 1]).max())
+    # rebuild_Sk keeps L/2 lags per direction; rebuild_corrL integrates them
+    sktot, nhalf = statistics.rebuild_Sk(x3, x3.shape, 3, 0.25, 1.5)
+    corrl = statistics.rebuild_corrL(sktot, nhalf, (0.5, 0.5, 0.5))
+    for d in range(3):
+        exp = O.corrl_from_sk(ref[d][:x3.shape[d] // 2], 0.5)
+        assert abs(corrl[d] - exp) <= 1e-12 * max(1.0, abs(exp))
+
+
+def test_lag_covariance_matches_oracle(hip):
+    from scarl_amd import statistics
+    rng = np.random.default_rng(22)
+    x = np.asfortranarray(rng.standard_normal((40, 33, 28)) + 2.0)
+    mu = float(x.mean())
+    cov = statistics.lag_covariance(x, mu, 33)
+    ref = O.lag_covariance(x, mu, 33)
+    assert np.array_equal(np.isnan(cov), np.isnan(ref))
+    ok = ~np.isnan(ref)
+    np.testing.assert_allclose(cov[ok], ref[ok], rtol=0, atol=1e-12 * np.abs(ref[ok]).max())
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_batch_lag_cov_matches_oracle(hip, prec):
+    from scarl_amd import generator
+    specs = [generator.FieldSpec(n=n, model=m, corrL=(6.0, 6.0, 6.0), seed=4, stream=i)
+             for i, (n, m) in enumerate((((64, 48, 40), 1), ((48, 48, 48), 3), ((32, 40, 24), 1)))]
+    with generator.Batch.ensemble(specs, prec) as b:
+        b.run()
+        cov = b.lag_covariance(33)
+        outs = [b.output(i).astype(np.float64) for i in range(3)]
+        mom = b.moments()
+    for i, x in enumerate(outs):
+        ref = O.lag_covariance(x, mom[i, 3], 33)
+        assert np.array_equal(np.isnan(cov[i]), np.isnan(ref))
+        ok = ~np.isnan(ref)
+        np.testing.assert_allclose(cov[i][ok], ref[ok], rtol=0, atol=1e-10 * np.abs(ref[ok]).max())
+
+
+def test_device_noise_covariance_in_confidence_band(hip):
+    """Device Philox noise, FP32 batch: the ensemble-mean per-axis covariance at lags 0-32 lies
+    within 4 standard errors of the target correlation C(l) minus the sample-mean bias
+    sum_r C(r)/N.  Standard error of one field's lag estimate ~ sqrt(2 sum_r C(r)^2 / N) (Gaussian
+    field), divided by sqrt(fields x axes)."""
+    from scarl_amd import generator
+    for model, L, edge, nf in ((1, 12.0, 96, 12), (3, 20.0, 192, 6)):
+        specs = [generator.FieldSpec(n=(edge,) * 3, model=model, corrL=(L, L, L), seed=77, stream=i)
+                 for i in range(nf)]
+        with generator.Batch.ensemble(specs, 32) as b:
+            b.run()
+            cov = b.lag_covariance(33)
+            mom = b.moments()
+        N = float(edge) ** 3
+        r = np.arange(-edge // 2, edge // 2)
+        R = np.sqrt(r[:, None, None] ** 2 + r[None, :, None] ** 2 + r[None, None, :] ** 2)
+        C = np.exp(-(R / L) ** 2) if model == 1 else np.exp(-R / L)
+        bias = C.sum() / N
+        se = np.sqrt(2.0 * np.sum(C ** 2) / N / (nf * 3))
+        target = np.exp(-(np.arange(33) / L) ** 2) if model == 1 else np.exp(-np.arange(33) / L)
+        est = cov.mean(axis=(0, 1))
+        assert np.all(np.abs(est - (target - bias)) <= 4 * se), (model, est - target + bias, se)
+        assert np.all(np.abs(mom[:, 3]) <= 4 * np.sqrt(bias))  # sample means: sd = sqrt(sum C / N)
+
+
+def test_normalize_field_matches_oracle(hip):
+    from scarl_amd import main_ScaRL, ScarlError
+    rng = np.random.default_rng(23)
+    x = np.asfortranarray(rng.standard_normal((30, 22, 17)) * 4.0 - 1.0)
+    ref, avg_r, sd_r = O.normalize_field(x)
+    y = x.copy(order="F")
+    avg, sd = main_ScaRL.normalize_field((0, 0, 0), (1, 1, 1), x.shape, (0, 0, 0), 0, 0, y)
+    assert abs(avg - avg_r) <= 1e-14 * abs(avg_r) + 1e-15 and abs(sd - sd_r) <= 1e-14 * sd_r
+    np.testing.assert_allclose(y, ref, rtol=0, atol=1e-13)
+    with pytest.raises(ScarlError):
+        main_ScaRL.normalize_field((0, 0, 0), (2, 1, 1), x.shape, (4, 0, 0), 0, 0, y)

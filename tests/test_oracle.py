@@ -348,7 +348,7 @@ def test_moments_kahan_vs_naive():
 
 def test_lag_sums_brute_force():
     x = np.asfortranarray(np.random.default_rng(2).standard_normal((5, 4, 3)))
-    s, c = O.lag_sums(x, 3)
+    s, c = O.lag_sums(x, 3)  # centre 0
     assert abs(s[0, 1] - np.sum(x[:-1] * x[1:])) < 1e-13 and c[0, 1] == 4 * 4 * 3
     assert abs(s[2, 2] - np.sum(x[:, :, :-2] * x[:, :, 2:])) < 1e-13 and c[2, 2] == 5 * 4 * 1
 
@@ -365,3 +365,67 @@ def test_build_sk_shape_and_zero_planes():
     de = (3.1415926535898 ** 2 / 2) * dk ** 2
     v = np.sqrt(np.exp(-(1 + 1 + 1) * de)) * np.sqrt(8.0) * 2 * np.sqrt(dk ** 3)
     assert abs(abs(sk[1, 1, 1]) - v) < 1e-6 * v
+
+
+# ---- Stat_ScaRL corrL estimator, per-axis covariance, normalize_field -------------------------
+def _circular_autocorr(u, axis):
+    """(1/N) sum_m u_m u_{(m+l) mod N} averaged over the other axes, by np.roll (brute force)."""
+    n = u.shape[axis]
+    other = tuple(a for a in range(u.ndim) if a != axis)
+    return np.array([np.mean(np.sum(u * np.roll(u, -l, axis=axis), axis=axis) / n) if other else
+                     np.sum(u * np.roll(u, -l, axis=axis)) / n for l in range(n)])
+
+
+def test_rebuild_sk_is_the_circular_autocorrelation():
+    """The FFT restatement (|FFT/N|^2, mean, unnormalised inverse) equals the direct periodic
+    autocorrelation of the tapered, normalised field: pins the FFT scalings and the mean."""
+    rng = np.random.default_rng(11)
+    x3 = np.asfortranarray(rng.standard_normal((12, 9, 7)) * 2.0 + 0.5)
+    sk = O.rebuild_sk_corrl(x3, 1.0, 0.5, 2.0)
+    for d in range(3):
+        shp = [1, 1, 1]
+        shp[d] = x3.shape[d]
+        u = (x3 - 0.5) / 2.0 * O.stat_taper(x3.shape[d]).reshape(shp)
+        np.testing.assert_allclose(sk[d], _circular_autocorr(u, d), rtol=0, atol=1e-13)
+    x2 = np.asfortranarray(rng.standard_normal((10, 6)))
+    sk2 = O.rebuild_sk_corrl(x2, 1.0, 0.0, 1.0)  # 2D: no taper
+    for d in range(2):
+        np.testing.assert_allclose(sk2[d], _circular_autocorr(x2, d), rtol=0, atol=1e-13)
+
+
+def test_stat_taper_as_written():
+    """r_tukey is the float 0.1 (> 0.1 in double): at ratio = 0.1 exactly the first cosine branch
+    applies and gives ~0, where a double 0.1 would give 1; t_0 = 0; beyond r the second branch
+    oscillates with period r in ratio (ratio = 0.55 sits on a maximum)."""
+    t = O.stat_taper(11)
+    assert t[0] < 1e-15 and t[1] < 1e-6
+    t21 = O.stat_taper(21)
+    assert abs(t21[11] - 1.0) < 1e-6 and abs(t21[20] - 0.0) < 1e-6
+    assert O.R_TUKEY == float(np.float32(0.1)) and O.R_TUKEY > 0.1
+
+
+def test_corrl_from_sk_closed_form():
+    # trapezoid rule over the kept lags: 2 * (sum - (first + last)/2) * xStep
+    assert O.corrl_from_sk([1.0, 0.5, 0.25, 0.0], 2.0) == 2.0 * (1.75 - 0.5) * 2.0
+
+
+def test_lag_covariance_centred_brute_force():
+    rng = np.random.default_rng(5)
+    x = np.asfortranarray(rng.standard_normal((7, 5, 4)) + 3.0)
+    mu = x.mean()
+    cov = O.lag_covariance(x, mu, 6)
+    y = x - mu
+    assert abs(cov[0, 2] - np.mean(y[:-2] * y[2:])) < 1e-14
+    assert abs(cov[1, 0] - np.mean(y * y)) < 1e-14
+    assert abs(cov[2, 3] - np.mean(y[:, :, :-3] * y[:, :, 3:])) < 1e-14
+    assert np.isnan(cov[2, 4]) and np.isnan(cov[1, 5]) and not np.isnan(cov[0, 5])
+
+
+def test_normalize_field_oracle_invariants():
+    """One rank: after normalisation the box has mean 0 and population std 1; with an overlap on a
+    side that is not last the box excludes it (maxPos = Np - Np_ovlp, main_ScaRL.f90:920-925)."""
+    rng = np.random.default_rng(8)
+    x = np.asfortranarray(rng.standard_normal((9, 8, 7)) * 3.0 + 2.0)
+    y, avg, sd = O.normalize_field(x)
+    assert abs(y.mean()) < 1e-14 and abs(y.std() - 1.0) < 1e-14
+    assert abs(avg - x.mean()) < 1e-14 and abs(sd - x.std()) < 1e-13
