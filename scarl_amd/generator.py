@@ -197,3 +197,36 @@ def timing_report():
 
 def timing_reset():
     A.call("scarl_hip_timing_reset")
+
+
+def seeded_tiling(gdim, nb, overlap, choices, seed):
+    """Per-axis subdomain starts and extents that tile [0, gdim) with neighbours overlapping by
+    `overlap` points: extents are drawn by seed from `choices` (sorted, evenly spaced), then fixed up
+    one step at a time (a random subdomain moves one choice up or down) until the extents sum to
+    gdim + (nb - 1) * overlap.  Returns (starts, exts), each a list of three lists."""
+    rng = np.random.default_rng(seed)
+    ch = sorted(int(c) for c in choices)
+    step = ch[1] - ch[0] if len(ch) > 1 else 0
+    starts, exts = [], []
+    for a in range(3):
+        n, g = int(nb[a]), int(gdim[a])
+        if n == 1:
+            starts.append([0])
+            exts.append([g])
+            continue
+        target = g + (n - 1) * overlap
+        if not (n * ch[0] <= target <= n * ch[-1]) or (step and (target - n * ch[0]) % step):
+            raise ValueError(f"axis {a}: {n} extents from {ch} cannot tile {g} with overlap {overlap}")
+        e = [int(v) for v in rng.choice(ch, size=n)]
+        while sum(e) != target:
+            i = int(rng.integers(n))
+            if sum(e) > target and e[i] > ch[0]:
+                e[i] -= step
+            elif sum(e) < target and e[i] < ch[-1]:
+                e[i] += step
+        s = [0]
+        for i in range(1, n):
+            s.append(int(s[-1] + e[i - 1] - overlap))
+        starts.append(s)
+        exts.append([int(v) for v in e])
+    return starts, exts

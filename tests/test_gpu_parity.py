@@ -366,6 +366,23 @@ def test_shino_fft(hip):
 
 
 # ---- Stat_ScaRL: corrL estimator, per-axis covariance; normalize_field ------------------------
+def _expected_lag_cov(n, model, L, nlag, nu=0.5):
+    """Exact expectation of the mean-centred per-axis lag covariance of the periodic Hermitian
+    generator: the sample mean is the k = 0 mode, so E = C_per(l e_a) - amp(0)^2 with C_per the
+    unnormalised C2R of amp^2; plus the large-N variance 2 sum_r C_per(r)^2 / N of one lag
+    estimate (Gaussian field) per axis."""
+    amp = O.amplitude_h(n, (1, 1, 1), model, L, nu)
+    Cp = O.c2r_3d(amp.astype(np.complex128) ** 2, n)
+    p0 = amp[0, 0, 0] ** 2
+    N = float(np.prod(n))
+    exp = np.full((3, nlag), np.nan)
+    for a in range(3):
+        m = min(nlag, n[a])
+        idx = [0, 0, 0]
+        for l in range(m):
+            idx[a] = l
+            exp[a, l] = Cp[tuple(idx)] - p0
+    return exp, 2.0 * np.sum(Cp ** 2) / N
 def test_rebuild_sk_fft_matches_oracle(hip):
     from scarl_amd import statistics
     rng = np.random.default_rng(21)
@@ -429,16 +446,13 @@ def test_device_noise_covariance_in_confidence_band(hip):
             b.run()
             cov = b.lag_covariance(33)
             mom = b.moments()
-        N = float(edge) ** 3
-        r = np.arange(-edge // 2, edge // 2)
-        R = np.sqrt(r[:, None, None] ** 2 + r[None, :, None] ** 2 + r[None, None, :] ** 2)
-        C = np.exp(-(R / L) ** 2) if model == 1 else np.exp(-R / L)
-        bias = C.sum() / N
-        se = np.sqrt(2.0 * np.sum(C ** 2) / N / (nf * 3))
-        target = np.exp(-(np.arange(33) / L) ** 2) if model == 1 else np.exp(-np.arange(33) / L)
+        exp, var1 = _expected_lag_cov((edge,) * 3, model, (L, L, L), 33)
+        se = np.sqrt(var1 / (nf * 3))
         est = cov.mean(axis=(0, 1))
-        assert np.all(np.abs(est - (target - bias)) <= 4 * se), (model, est - target + bias, se)
-        assert np.all(np.abs(mom[:, 3]) <= 4 * np.sqrt(bias))  # sample means: sd = sqrt(sum C / N)
+        assert np.all(np.abs(est - exp.mean(axis=0)) <= 4 * se), (model, est - exp.mean(axis=0), se)
+        # the sample mean is the k = 0 mode: sd amp(0)
+        a0 = O.amplitude_h((edge,) * 3, (1, 1, 1), model, (L, L, L))[0, 0, 0]
+        assert np.all(np.abs(mom[:, 3]) <= 4 * a0)
 
 
 def test_normalize_field_matches_oracle(hip):
@@ -452,3 +466,106 @@ def test_normalize_field_matches_oracle(hip):
     np.testing.assert_allclose(y, ref, rtol=0, atol=1e-13)
     with pytest.raises(ScarlError):
         main_ScaRL.normalize_field((0, 0, 0), (2, 1, 1), x.shape, (4, 0, 0), 0, 0, y)
+
+
+# ---- the workloads of the benchmark configurations --------------------------------------------
+def test_config1_2d_gaussian_against_numpy_irfftn(hip):
+    """2D 256x256, unit spacing, Gaussian C = exp(-(r/8)^2), sigma 1, standard-normal marginal,
+    Hermitian N(0,1) noise of seed 1, FP64: against numpy's irfftn of amp * W (unnormalised)."""
+    from scarl_amd import generator
+    n = (256, 256, 1)
+    spec = generator.FieldSpec(n=n, model=1, corrL=(8.0, 8.0, 8.0), seed=1, margi=1, avg=0.0, std_dev=1.0)
+    W = O.noise_hermitian(n, 1, 0)
+    gpu = generator.gen_field_h(spec, 64, W)
+    amp = O.amplitude_h(n, (1, 1, 1), 1, (8.0, 8.0, 8.0))
+    ref = np.fft.irfftn((amp * W)[:, :, 0], s=(256, 256), axes=(1, 0)) * (256 * 256)
+    assert rel_l2(gpu[:, :, 0], ref) <= FP64_TOL
+    _say(f"config 1: mean {gpu.mean():.4f} variance {gpu.var():.4f}")
+    assert abs(gpu.mean()) < 0.2 and abs(gpu.var() - 1.0) < 0.3  # ~ (256/8)^2 = 1024 correlation areas
+
+
+def test_config4_blend_1024x1024x256(hip):
+    """Global 1024x1024x256 from 8x8x2 subdomains of seeded extents (128-192, multiples of 16),
+    16-point overlaps, cosine PoU, exponential L = 24, lognormal mean 1 CoV 0.5, FP32, one batch.
+    The blended Gaussian field equals the gather-sum of sqrt(w) times the same subdomains generated
+    as an ensemble, and the lognormal field's moments sit in their bands."""
+    from scarl_amd import generator
+    gdim = (1024, 1024, 256)
+    starts, exts = generator.seeded_tiling(gdim, (8, 8, 2), 16, (128, 144, 160, 176, 192), 2024)
+    base = dict(model=3, corrL=(24.0, 24.0, 24.0), seed=31)
+    with generator.Batch.blend(gdim, starts, exts, generator.FieldSpec(n=(2, 2, 2), **base), 32) as b:
+        b.run(use_graph=True)
+        b.sync()
+        G = b.output(-1)
+    specs, st, ws = [], [], []
+    sid = 0
+    for k in range(2):
+        for j in range(8):
+            for i in range(8):
+                pos, n = (i, j, k), (exts[0][i], exts[1][j], exts[2][k])
+                specs.append(generator.FieldSpec(n=n, stream=sid, **base))
+                st.append((starts[0][i], starts[1][j], starts[2][k]))
+                w = []
+                for a in range(3):
+                    p = pos[a]
+                    lo = starts[a][p - 1] + exts[a][p - 1] - starts[a][p] if p > 0 else 0
+                    hi = starts[a][p] + exts[a][p] - starts[a][p + 1] if p + 1 < len(starts[a]) else 0
+                    w.append(O.pou_weights_1d(n[a], lo, hi, lo > 0, hi > 0))
+                ws.append(w)
+                sid += 1
+    with generator.Batch.ensemble(specs, 32) as e:
+        e.run()
+        fields = [e.output(i).astype(np.float64) for i in range(len(specs))]
+    ref = O.blend_global(fields, st, gdim, ws)
+    err = rel_l2(G, ref)
+    _say(f"config 4: blend vs gather-sum rel L2 {err:.2e}")
+    assert err <= 1e-6
+    with generator.Batch.blend(gdim, starts, exts, generator.FieldSpec(n=(2, 2, 2), margi=2, avg=1.0, std_dev=0.5,
+                                                                       **base), 32) as b:
+        b.run(use_graph=True)
+        b.sync()
+        mom = b.moments()[0]
+    _say(f"config 4: lognormal mean {mom[3]:.4f} std {mom[4]:.4f}")
+    assert mom[2] == 1024 * 1024 * 256
+    # ~770 correlation volumes: the underlying Gaussian mean has sd sqrt(sum C / N) ~ 0.036, the
+    # lognormal mean ~ 0.47 of that; bands of 4 sd
+    assert abs(mom[3] - 1.0) < 0.07 and abs(mom[4] / mom[3] - 0.5) < 0.1
+
+
+def test_config5_ensemble_256(hip):
+    """256 realizations, edges drawn by seed from {64, 96, 128, 160, 192}, half Gaussian (L = 12) and
+    half exponential (L = 20), standard-normal marginal, FP32, one batch: per-axis covariance at lags
+    0-32 in the confidence band of each model; two realizations against the FP64 oracle (field and
+    lag covariance)."""
+    from scarl_amd import generator
+    rng = np.random.default_rng(5)
+    edges = rng.choice([64, 96, 128, 160, 192], size=256)
+    specs = [generator.FieldSpec(n=(int(e),) * 3, model=1 if i < 128 else 3, corrL=(12.0,) * 3 if i < 128 else (20.0,) * 3,
+                                 seed=55, stream=i, margi=1, avg=0.0, std_dev=1.0) for i, e in enumerate(edges)]
+    with generator.Batch.ensemble(specs, 32) as b:
+        b.run(use_graph=True)
+        b.sync()
+        cov = b.lag_covariance(33)
+        mom = b.moments()
+        small = [i for i in range(256) if edges[i] == 64]
+        picks = [small[0], [i for i in small if i >= 128][0]]
+        outs = {i: b.output(i) for i in picks}
+    assert mom[:, 2].sum() == float(np.sum(edges.astype(np.float64) ** 3))
+    stats = {}  # (edge, model) -> exact expectation and one-axis variance
+    for model, L, sel in ((1, 12.0, slice(0, 128)), (3, 20.0, slice(128, 256))):
+        dev, var = [], []
+        for i in range(256)[sel]:
+            e = int(edges[i])
+            if (e, model) not in stats:
+                stats[(e, model)] = _expected_lag_cov((e,) * 3, model, (L, L, L), 33)
+            exp, v = stats[(e, model)]
+            dev.append(cov[i].mean(axis=0) - exp.mean(axis=0))
+            var.append(v / 3)
+        band = 4 * np.sqrt(np.sum(var)) / len(var)
+        assert np.all(np.abs(np.mean(dev, axis=0)) <= band), (model, np.mean(dev, axis=0), band)
+    for i in picks:
+        sp = specs[i]
+        ref = O.gen_mode_h(sp.n, (1, 1, 1), sp.model, sp.corrL, 0.5, seed=55, stream=i)
+        assert rel_l2(outs[i], ref) <= FP32_TOL
+        rc = O.lag_covariance(outs[i].astype(np.float64), mom[i, 3], 33)
+        np.testing.assert_allclose(cov[i], rc, rtol=0, atol=1e-10 * np.abs(rc).max())
