@@ -19,6 +19,26 @@ struct cplx {
     T x, y;
 };
 
+// Global-memory accesses through address space 1.  Pointers held in descriptors are generic to
+// the compiler, whose FLAT accesses also count against the LDS counter (every wait for one then
+// waits for LDS traffic too); these compile to global_load / global_store.
+#define SCARL_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ cplx<T> gload(const cplx<T>* p) {
+    const SCARL_GLOBAL T* q = (const SCARL_GLOBAL T*)p;
+    return {q[0], q[1]};
+}
+template <typename T>
+__device__ __forceinline__ void gstore(cplx<T>* p, cplx<T> v) {
+    SCARL_GLOBAL T* q = (SCARL_GLOBAL T*)p;
+    q[0] = v.x;
+    q[1] = v.y;
+}
+template <typename T>
+__device__ __forceinline__ T gload1(const T* p) { return *(const SCARL_GLOBAL T*)p; }
+template <typename T>
+__device__ __forceinline__ void gstore1(T* p, T v) { *(SCARL_GLOBAL T*)p = v; }
+
 template <typename T>
 __device__ __forceinline__ cplx<T> operator+(cplx<T> a, cplx<T> b) { return {a.x + b.x, a.y + b.y}; }
 template <typename T>

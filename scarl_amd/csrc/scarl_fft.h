@@ -293,7 +293,7 @@ template <typename T>
 __device__ __forceinline__ T apply_margi(T v, int margi, T mu, T sd) {
     if (margi == FOM_GAUSSIAN) return v * sd + mu;
     if (margi == FOM_LOGNORMAL) {
-        if constexpr (sizeof(T) == 4) return expf(v * sd + mu);
+        if constexpr (sizeof(T) == 4) return __builtin_amdgcn_exp2f((v * sd + mu) * 1.44269504088896340736f);
         else return exp(v * sd + mu);
     }
     return v;

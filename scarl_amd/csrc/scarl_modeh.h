@@ -66,7 +66,7 @@ template <typename T>
 __device__ __forceinline__ cplx<T> generate_h(const FieldH& f, int kx, int ky, int kz) {
     cplx<T> w;
     if (f.src_mode == SRC_HOST_NOISE)
-        w = reinterpret_cast<const cplx<T>*>(f.W)[kx + (int64_t)f.h1 * (ky + (int64_t)f.n[1] * kz)];
+        w = gload(reinterpret_cast<const cplx<T>*>(f.W) + (kx + (int64_t)f.h1 * (ky + (int64_t)f.n[1] * kz)));
     else w = noise_h<T>(f, kx, ky, kz);
     const T a = amplitude_h<T>(f, kx, ky, kz);
     return {a * w.x, a * w.y};
@@ -107,9 +107,9 @@ __global__ __launch_bounds__(kBlock) void k_h_strided(const FieldH* __restrict__
         if constexpr (GEN) {
             if (gen) return (axis == 2) ? generate_h<T>(f, c.kx, c.k2, n) : generate_h<T>(f, c.kx, n, c.k2);
         }
-        return ws[c.base + n * S];
+        return gload(ws + (c.base + n * S));
     };
-    auto gst = [&](const Ctx& c, int n, cplx<T> v) { ws[c.base + n * S] = v; };
+    auto gst = [&](const Ctx& c, int n, cplx<T> v) { gstore(ws + (c.base + n * S), v); };
     // stage-in of the generating pass: along z, neighbouring kz share one Philox call
     auto gin = [&](const Ctx& c, int b, int tpp, auto&& put) {
         const int N = pl.N;
@@ -217,7 +217,7 @@ __global__ __launch_bounds__(kBlock) void k_h_xpass(const FieldH* __restrict__ f
         return c;
     };
     auto gld = [&](const Ctx& c, int k) -> cplx<T> {
-        cplx<T> a = c.X[k], bb = k ? c.X[M - k] : *c.Xn;
+        cplx<T> a = gload(c.X + k), bb = gload(k ? c.X + (M - k) : c.Xn);
         if (k == 0) {
             a.y = T(0);
             bb.y = T(0);
@@ -238,8 +238,7 @@ __global__ __launch_bounds__(kBlock) void k_h_xpass(const FieldH* __restrict__ f
                 ms += d0 + d1;
                 ms2 += d0 * d0 + d1 * d1;
             }
-            if constexpr (sizeof(T) == 4) *reinterpret_cast<float2*>(o + 2 * k) = make_float2(r0, r1);
-            else *reinterpret_cast<double2*>(o + 2 * k) = make_double2(r0, r1);
+            gstore(reinterpret_cast<cplx<T>*>(o + 2 * k), cplx<T>{r0, r1});
         } else {
         const T wyz = c.wyz;
         const int myz = c.myz;
@@ -251,7 +250,7 @@ __global__ __launch_bounds__(kBlock) void k_h_xpass(const FieldH* __restrict__ f
             const int msk = myz | (in_overlap(i, n1, f.nov_lo[0], f.nov_hi[0]) ? 1 : 0);
             const bool first = (f.color & msk) == 0;
             const bool last = (f.color & msk) == msk;
-            if (!first) val += o[i];
+            if (!first) val += gload1(o + i);
             if (last) {
                 val = apply_margi<T>(val, f.margi, mu, sd);
                 if (mom) {
@@ -260,7 +259,7 @@ __global__ __launch_bounds__(kBlock) void k_h_xpass(const FieldH* __restrict__ f
                     ms2 += d * d;
                 }
             }
-            o[i] = val;
+            gstore1(o + i, val);
         }
         }
     };
