@@ -13,7 +13,7 @@ namespace scarl {
 __device__ __forceinline__ double mode_r_amp(const FieldR& f, int i, int j, int k) {
     // SRC_F/fftw_ScaRL.f90:305-370 (kPoints, Sk product in axis order, sqrt(S) cos(2 PI phi))
     const int64_t pidx = i + (int64_t)f.m[0] * (j + (int64_t)f.m[1] * k);
-    const double ph = (f.src_mode == SRC_HOST_NOISE) ? f.phi[pidx]
+    const double ph = (f.src_mode == SRC_HOST_NOISE) ? gload1(f.phi + pidx)
                                                      : uniform53(static_cast<uint64_t>(pidx), f.seed, f.stream);
     const double k0 = ((double)i * f.dk[0]) / f.corrL[0];
     const double k1 = ((double)j * f.dk[1]) / f.corrL[1];
@@ -26,11 +26,11 @@ __device__ __forceinline__ double mode_r_amp(const FieldR& f, int i, int j, int 
 }
 
 template <int AXIS, class Fft>
-__global__ __launch_bounds__(kBlock) void k_r_pass(const FieldR* __restrict__ fp, const PlanDev* __restrict__ plans,
+__global__ __launch_bounds__(kBlock) void k_r_pass(const FieldR fp, const PlanDev* __restrict__ plans,
                                                     const cplx<double>* __restrict__ pool) {
     HIP_DYNAMIC_SHARED(char, smem)
     cplx<double>* lds = reinterpret_cast<cplx<double>*>(smem);
-    const FieldR& f = *fp;
+    const FieldR& f = fp;
     const PlanDev& pl = plans[f.plan[AXIS]];
     const int n1 = f.n[0], n2 = f.n[1], n3 = f.n[2];
     const int m1 = f.m[0], m2 = f.m[1], m3 = f.m[2];
@@ -78,19 +78,19 @@ __global__ __launch_bounds__(kBlock) void k_r_pass(const FieldR* __restrict__ fp
     auto rin = [&](const Ctx& c, int e, int j) -> double {
         if (j >= m) return 0.0;
         if constexpr (AXIS == 0) return mode_r_amp(f, j, c.jy[e], c.kz[e]);
-        else if constexpr (AXIS == 1) return f.tmp1[c.in[e] + (int64_t)n1 * j];
-        else return f.tmp2[c.in[e] + (int64_t)n1 * n2 * j];
+        else if constexpr (AXIS == 1) return gload1(f.tmp1 + (c.in[e] + (int64_t)n1 * j));
+        else return gload1(f.tmp2 + (c.in[e] + (int64_t)n1 * n2 * j));
     };
     auto rout = [&](const Ctx& c, int e, int o, double v) {
         if constexpr (AXIS == 0) {
-            f.tmp1[c.out[e] + o] = v;
+            gstore1(f.tmp1 + (c.out[e] + o), v);
         } else if constexpr (AXIS == 1) {
-            f.tmp2[c.out[e] + (int64_t)n1 * o] = v;
+            gstore1(f.tmp2 + (c.out[e] + (int64_t)n1 * o), v);
         } else {
             double r = v;
             if (f.margi == FOM_GAUSSIAN) r = r * f.sd + f.mu;
             else if (f.margi == FOM_LOGNORMAL) r = exp(r * f.sd + f.mu);
-            f.out[c.out[e] + (int64_t)n1 * n2 * o] = r;
+            gstore1(f.out + (c.out[e] + (int64_t)n1 * n2 * o), r);
         }
     };
     auto gld = [&](const Ctx& c, int j) -> cplx<double> {
@@ -236,12 +236,12 @@ __global__ __launch_bounds__(kBlock) void k_c2c_xcrop(const PlanDev* __restrict_
 }
 
 template <int AXIS, class F>
-static void r_go(const FieldR* f, const PlanDev* plans, const cplx<double>* p, int nblocks, size_t lds, hipStream_t s) {
+static void r_go(const FieldR& f, const PlanDev* plans, const cplx<double>* p, int nblocks, size_t lds, hipStream_t s) {
     hipLaunchKernelGGL((k_r_pass<AXIS, F>), dim3(nblocks), dim3(kBlock), lds, s, f, plans, p);
 }
 
 template <int AXIS>
-static void r_dispatch(int spec, const FieldR* f, const PlanDev* plans, const cplx<double>* p, int nblocks,
+static void r_dispatch(int spec, const FieldR& f, const PlanDev* plans, const cplx<double>* p, int nblocks,
                        size_t lds, hipStream_t s) {
     constexpr int LAY = AXIS == 0 ? LAYOUT_BF : LAYOUT_PF;
     switch (spec) {
@@ -255,7 +255,7 @@ static void r_dispatch(int spec, const FieldR* f, const PlanDev* plans, const cp
     }
 }
 
-hipError_t launch_r_pass(int axis, int spec, const FieldR* f, const PlanDev* plans, const void* pool, int nblocks,
+hipError_t launch_r_pass(int axis, int spec, const FieldR& f, const PlanDev* plans, const void* pool, int nblocks,
                          size_t lds_bytes, hipStream_t s) {
     const cplx<double>* p = static_cast<const cplx<double>*>(pool);
     if (nblocks <= 0) return hipSuccess;

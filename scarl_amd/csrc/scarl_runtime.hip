@@ -51,7 +51,7 @@ static hipError_t launch_h_xpass(bool fp64, int spec, int ecap, bool blend, cons
 hipError_t launch_mom(const FieldH* fields, int nfields, const int2* field_tiles, const MomPartial* partials,
                       MomPartial* field_mom, const double* slot_count, int nslots, double* slot_out, hipStream_t s);
 hipError_t launch_spectrum_sum(const FieldH& f, MomPartial* out, int nblocks, hipStream_t s);
-hipError_t launch_r_pass(int axis, int spec, const FieldR* f, const PlanDev* plans, const void* pool, int nblocks,
+hipError_t launch_r_pass(int axis, int spec, const FieldR& f, const PlanDev* plans, const void* pool, int nblocks,
                          size_t lds_bytes, hipStream_t s);
 hipError_t launch_marginal(double* x, int64_t n, int margi, double mu, double sd, hipStream_t s);
 hipError_t launch_pou(double* x, int n1, int n2, int n3, int3 lo, int3 hi, hipStream_t s);
@@ -836,7 +836,6 @@ struct ModeRCache {
     std::unique_ptr<PlanSet> ps;
     int plan[3] = {0, 0, 0};
     size_t lds[3] = {0, 0, 0};
-    FieldR* d_field = nullptr;
 };
 static ModeRCache g_r;
 
@@ -857,7 +856,6 @@ static int mode_r_prepare(const int32_t n[3], const int32_t m[3]) {
         g_r.n[a] = n[a];
         g_r.m[a] = m[a];
     }
-    if (!g_r.d_field) SCARL_TRY(dmalloc((void**)&g_r.d_field, sizeof(FieldR)));
     g_r.ps = std::move(ps);
     return 0;
 }
@@ -897,7 +895,7 @@ static int mode_r_run(double* d_out, const int32_t n[3], const int32_t m[3], con
     f.tmp1 = static_cast<double*>(p1);
     f.tmp2 = static_cast<double*>(p2);
     f.out = d_out;
-    HIP_TRY(hipMemcpyAsync(g_r.d_field, &f, sizeof f, hipMemcpyHostToDevice, g_ctx.stream));
+    // the descriptor travels by value in the kernel arguments (no host buffer outlives the call)
     const int64_t npen_real[3] = {(int64_t)m[1] * m[2], (int64_t)n[0] * m[2], (int64_t)n[0] * n[1]};
     static const char* names[3] = {"r_xpass", "r_ypass", "r_zpass"};
     for (int a = 0; a < 3; ++a) {
@@ -910,7 +908,7 @@ static int mode_r_run(double* d_out, const int32_t n[3], const int32_t m[3], con
             continue;
         }
         SCARL_TRY(timed(names[a], [&] {
-            return launch_r_pass(a, pl.spec, g_r.d_field, g_r.ps->d_plans, g_r.ps->d_pool, (int)nblocks, g_r.lds[a],
+            return launch_r_pass(a, pl.spec, f, g_r.ps->d_plans, g_r.ps->d_pool, (int)nblocks, g_r.lds[a],
                                  g_ctx.stream);
         }));
     }
@@ -943,8 +941,6 @@ int scarl_hip_finalize(void) {
     }
     g_ctx.pending.clear();
     g_r.ps.reset();
-    if (g_r.d_field) hipFree(g_r.d_field);
-    g_r.d_field = nullptr;
     memset(g_r.n, 0, sizeof g_r.n);
     hipStreamDestroy(g_ctx.stream);
     hipStreamDestroy(g_ctx.gen_stream);
