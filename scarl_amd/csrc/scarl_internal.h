@@ -96,7 +96,7 @@ struct Tile {
 // plan.  prec: 4 = FP32, 8 = FP64; layout: LAYOUT_PF (strided y/z passes) or LAYOUT_BF (x pass);
 // a tile holds 256/TPP pencils and every tile stays within 64 KiB of LDS.
 #define SCARL_FIXED_SPECS(X)                \
-    X(1, 4, LAYOUT_PF, 512, 32, 16, 16)     \
+    X(1, 4, LAYOUT_PF, 512, 16, 32, 16)     \
     X(2, 4, LAYOUT_PF, 256, 16, 16, 16)     \
     X(3, 4, LAYOUT_PF, 128, 16, 8, 8)       \
     X(4, 4, LAYOUT_PF, 64, 8, 8, 8)         \
