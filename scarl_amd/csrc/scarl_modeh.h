@@ -170,12 +170,15 @@ __global__ __launch_bounds__(kBlock) void k_h_strided(const FieldH* __restrict__
     Fft::template run<T, LAYOUT_PF, GEN, false>(pl, pool, lds, npen, prep, gld, gst, gin);
 }
 
-// x pass: half-length complex FFT of Z_k = (X_k + X*_{M-k}) + i e^{2 pi i k/n1} (X_k - X*_{M-k})
-// (imaginary parts of X_0 and X_M ignored), x_{2k} = Re z_k, x_{2k+1} = Im z_k; then
+// x pass: for even n1 a half-length complex FFT of
+// Z_k = (X_k + X*_{M-k}) + i e^{2 pi i k/n1} (X_k - X*_{M-k}) (imaginary parts of X_0 and X_M
+// ignored), x_{2k} = Re z_k, x_{2k+1} = Im z_k; for odd n1 a full-length complex FFT of the
+// Hermitian extension (X_k for k < h1, conj X_{n1-k} above; imaginary part of X_0 ignored),
+// x_k = Re z_k; then
 //   no blend: marginal, store, moments;
 //   blend:    value * sqrt(w_PoU); first owner (colour order) stores, later owners add, the last
 //             owner applies the marginal and takes the moments.
-template <typename T, class Fft, bool BLEND>
+template <typename T, class Fft, bool BLEND, bool FULL>
 __global__ __launch_bounds__(kBlock) void k_h_xpass(const FieldH* __restrict__ fields, const Tile* __restrict__ tiles,
                                                      int tile0, const PlanDev* __restrict__ plans,
                                                      const cplx<T>* __restrict__ pool,
@@ -186,7 +189,8 @@ __global__ __launch_bounds__(kBlock) void k_h_xpass(const FieldH* __restrict__ f
     const Tile tl = tiles[tix];
     const FieldH& f = fields[tl.field];
     const PlanDev& pl = plans[f.plan_x];
-    const int n1 = f.n[0], n2 = f.n[1], n3 = f.n[2], M = n1 / 2;
+    const int n1 = f.n[0], n2 = f.n[1], n3 = f.n[2], M = n1 / 2, h1 = f.h1;
+    constexpr bool full = FULL;  // odd n1 (full-length C2C over the Hermitian extension)
     const int64_t nrows = (int64_t)n2 * n3;
     const int npen = (int)min((int64_t)pl.T, nrows - (int64_t)tl.pencil0);
     const cplx<T>* ws = reinterpret_cast<const cplx<T>*>(f.ws);
@@ -197,8 +201,8 @@ __global__ __launch_bounds__(kBlock) void k_h_xpass(const FieldH* __restrict__ f
     const bool mom = f.mom_slot >= 0;
 
     struct Ctx {
-        const cplx<T>* X;   // row (kx < M)
-        const cplx<T>* Xn;  // its kx = M element in the Nyquist plane
+        const cplx<T>* X;   // row (kx < wp)
+        const cplx<T>* Xn;  // its kx = M element in the Nyquist plane (even n1)
         T* o;
         T wyz;     // PoU weight of the row (blend)
         int myz;   // overlap mask bits of the row (blend)
@@ -206,7 +210,7 @@ __global__ __launch_bounds__(kBlock) void k_h_xpass(const FieldH* __restrict__ f
     auto prep = [&](int p) -> Ctx {
         const int row = tl.pencil0 + p;
         const int ky = row % n2, kz = row / n2;
-        Ctx c = {ws + (int64_t)ky * M + kz * f.wz, ws + f.wz * n3 + row, out + ky * f.out_ld1 + kz * f.out_ld2,
+        Ctx c = {ws + (int64_t)ky * f.wp + kz * f.wz, ws + f.wz * n3 + row, out + ky * f.out_ld1 + kz * f.out_ld2,
                  T(1), 0};
         if (BLEND) {
             c.wyz = pou_sqrt_weight<T>(ky, n2, f.nov_lo[1], f.nov_hi[1]) *
@@ -217,6 +221,15 @@ __global__ __launch_bounds__(kBlock) void k_h_xpass(const FieldH* __restrict__ f
         return c;
     };
     auto gld = [&](const Ctx& c, int k) -> cplx<T> {
+        if (full) {
+            if (k < h1) {
+                cplx<T> a = gload(c.X + k);
+                if (k == 0) a.y = T(0);
+                return a;
+            }
+            const cplx<T> a = gload(c.X + (n1 - k));
+            return {a.x, -a.y};
+        }
         cplx<T> a = gload(c.X + k), bb = gload(k ? c.X + (M - k) : c.Xn);
         if (k == 0) {
             a.y = T(0);
@@ -230,6 +243,16 @@ __global__ __launch_bounds__(kBlock) void k_h_xpass(const FieldH* __restrict__ f
     auto gst = [&](const Ctx& c, int k, cplx<T> z) {
         T* o = c.o;
         if constexpr (!BLEND) {
+            if (full) {
+                const T r = apply_margi<T>(z.x, f.margi, mu, sd);
+                if (mom) {
+                    const double d = r;
+                    ms += d;
+                    ms2 += d * d;
+                }
+                gstore1(o + k, r);
+                return;
+            }
             const T r0 = apply_margi<T>(z.x, f.margi, mu, sd), r1 = apply_margi<T>(z.y, f.margi, mu, sd);
             if (mom) {
                 // a thread sums at most a few dozen values in plain FP64; the block, tile and
@@ -242,9 +265,9 @@ __global__ __launch_bounds__(kBlock) void k_h_xpass(const FieldH* __restrict__ f
         } else {
         const T wyz = c.wyz;
         const int myz = c.myz;
-#pragma unroll
-        for (int e = 0; e < 2; ++e) {
-            const int i = 2 * k + e;
+        const int ne = full ? 1 : 2;
+        for (int e = 0; e < ne; ++e) {
+            const int i = full ? k : 2 * k + e;
             const T w = wyz * pou_sqrt_weight<T>(i, n1, f.nov_lo[0], f.nov_hi[0]);
             T val = (e ? z.y : z.x) * sqrt(w);
             const int msk = myz | (in_overlap(i, n1, f.nov_lo[0], f.nov_hi[0]) ? 1 : 0);
@@ -287,15 +310,15 @@ struct ModeHDispatch {
             hipLaunchKernelGGL((k_h_strided<T, false, F>), dim3(ntiles), dim3(kBlock), lds, s, fields, tiles, tile0,
                                plans, p, axis);
     }
-    template <class F>
+    template <class F, bool FULL = false>
     static void xpass(bool blend, const FieldH* fields, const Tile* tiles, int tile0, int ntiles,
                       const PlanDev* plans, const cplx<T>* p, MomPartial* partials, size_t lds, hipStream_t s) {
         if (blend)
-            hipLaunchKernelGGL((k_h_xpass<T, F, true>), dim3(ntiles), dim3(kBlock), lds, s, fields, tiles, tile0,
+            hipLaunchKernelGGL((k_h_xpass<T, F, true, FULL>), dim3(ntiles), dim3(kBlock), lds, s, fields, tiles, tile0,
                                plans, p, partials);
         else
-            hipLaunchKernelGGL((k_h_xpass<T, F, false>), dim3(ntiles), dim3(kBlock), lds, s, fields, tiles, tile0,
-                               plans, p, partials);
+            hipLaunchKernelGGL((k_h_xpass<T, F, false, FULL>), dim3(ntiles), dim3(kBlock), lds, s, fields, tiles,
+                               tile0, plans, p, partials);
     }
     static hipError_t run_strided(int axis, bool gen, int spec, int ecap, const FieldH* fields, const Tile* tiles,
                                   int tile0, int ntiles, const PlanDev* plans, const void* pool, size_t lds,
@@ -316,12 +339,18 @@ struct ModeHDispatch {
         }
         return hipGetLastError();
     }
-    static hipError_t run_xpass(int spec, int ecap, bool blend, const FieldH* fields, const Tile* tiles, int tile0,
-                                int ntiles, const PlanDev* plans, const void* pool, MomPartial* partials, size_t lds,
-                                hipStream_t s) {
+    // full: odd n1, whose lengths only the generic kernel plans
+    static hipError_t run_xpass(int spec, int ecap, bool blend, bool full, const FieldH* fields, const Tile* tiles,
+                                int tile0, int ntiles, const PlanDev* plans, const void* pool, MomPartial* partials,
+                                size_t lds, hipStream_t s) {
         if (ntiles <= 0) return hipSuccess;
         if (lds < 4 * kBlock * sizeof(double)) lds = 4 * kBlock * sizeof(double);
         const cplx<T>* p = static_cast<const cplx<T>*>(pool);
+        if (full) {
+            if (ecap == E2) xpass<GenericFft<E2>, true>(blend, fields, tiles, tile0, ntiles, plans, p, partials, lds, s);
+            else xpass<GenericFft<E1>, true>(blend, fields, tiles, tile0, ntiles, plans, p, partials, lds, s);
+            return hipGetLastError();
+        }
         switch (spec) {
 #define SCARL_CASE(ID, P, L, N, R0, R1, TPP)                                                                  \
     case ID:                                                                                                  \
@@ -349,10 +378,10 @@ struct ModeHDispatch {
         return ModeHDispatch<T, PREC, E1, E2>::run_strided(axis, gen, spec, ecap, fields, tiles, tile0, ntiles, \
                                                            plans, pool, lds, s);                           \
     }                                                                                                      \
-    hipError_t launch_h_xpass_##SFX(int spec, int ecap, bool blend, const FieldH* fields, const Tile* tiles, \
-                                    int tile0, int ntiles, const PlanDev* plans, const void* pool,         \
-                                    MomPartial* partials, size_t lds, hipStream_t s) {                     \
-        return ModeHDispatch<T, PREC, E1, E2>::run_xpass(spec, ecap, blend, fields, tiles, tile0, ntiles, plans, \
-                                                         pool, partials, lds, s);                          \
+    hipError_t launch_h_xpass_##SFX(int spec, int ecap, bool blend, bool full, const FieldH* fields,       \
+                                    const Tile* tiles, int tile0, int ntiles, const PlanDev* plans,        \
+                                    const void* pool, MomPartial* partials, size_t lds, hipStream_t s) {   \
+        return ModeHDispatch<T, PREC, E1, E2>::run_xpass(spec, ecap, blend, full, fields, tiles, tile0, ntiles, \
+                                                         plans, pool, partials, lds, s);                   \
     }                                                                                                      \
     }

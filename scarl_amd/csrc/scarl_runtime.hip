@@ -29,10 +29,10 @@ hipError_t launch_h_strided_f32(int axis, bool gen, int spec, int ecap, const Fi
 hipError_t launch_h_strided_f64(int axis, bool gen, int spec, int ecap, const FieldH* fields, const Tile* tiles,
                                 int tile0, int ntiles, const PlanDev* plans, const void* pool, size_t lds,
                                 hipStream_t s);
-hipError_t launch_h_xpass_f32(int spec, int ecap, bool blend, const FieldH* fields, const Tile* tiles, int tile0,
+hipError_t launch_h_xpass_f32(int spec, int ecap, bool blend, bool full, const FieldH* fields, const Tile* tiles, int tile0,
                               int ntiles, const PlanDev* plans, const void* pool, MomPartial* partials, size_t lds,
                               hipStream_t s);
-hipError_t launch_h_xpass_f64(int spec, int ecap, bool blend, const FieldH* fields, const Tile* tiles, int tile0,
+hipError_t launch_h_xpass_f64(int spec, int ecap, bool blend, bool full, const FieldH* fields, const Tile* tiles, int tile0,
                               int ntiles, const PlanDev* plans, const void* pool, MomPartial* partials, size_t lds,
                               hipStream_t s);
 
@@ -42,11 +42,11 @@ static hipError_t launch_h_strided(bool fp64, int axis, bool gen, int spec, int 
     return fp64 ? launch_h_strided_f64(axis, gen, spec, ecap, fields, tiles, tile0, ntiles, plans, pool, lds, s)
                 : launch_h_strided_f32(axis, gen, spec, ecap, fields, tiles, tile0, ntiles, plans, pool, lds, s);
 }
-static hipError_t launch_h_xpass(bool fp64, int spec, int ecap, bool blend, const FieldH* fields, const Tile* tiles,
+static hipError_t launch_h_xpass(bool fp64, int spec, int ecap, bool blend, bool full, const FieldH* fields, const Tile* tiles,
                                  int tile0, int ntiles, const PlanDev* plans, const void* pool, MomPartial* partials,
                                  size_t lds, hipStream_t s) {
-    return fp64 ? launch_h_xpass_f64(spec, ecap, blend, fields, tiles, tile0, ntiles, plans, pool, partials, lds, s)
-                : launch_h_xpass_f32(spec, ecap, blend, fields, tiles, tile0, ntiles, plans, pool, partials, lds, s);
+    return fp64 ? launch_h_xpass_f64(spec, ecap, blend, full, fields, tiles, tile0, ntiles, plans, pool, partials, lds, s)
+                : launch_h_xpass_f32(spec, ecap, blend, full, fields, tiles, tile0, ntiles, plans, pool, partials, lds, s);
 }
 hipError_t launch_mom(const FieldH* fields, int nfields, const int2* field_tiles, const MomPartial* partials,
                       MomPartial* field_mom, const double* slot_count, int nslots, double* slot_out, hipStream_t s);
@@ -491,7 +491,6 @@ namespace scarl {
 static int validate_desc(const scarl_field_desc& d) {
     for (int a = 0; a < 3; ++a)
         if (d.n[a] < 1) return fail(SCARL_E_INVALID, "extent n[%d] = %d must be >= 1", a, d.n[a]);
-    if (d.n[0] % 2) return fail(SCARL_E_UNSUPPORTED, "Hermitian mode needs an even x extent (n[0] = %d)", d.n[0]);
     if (d.n[0] < 2) return fail(SCARL_E_INVALID, "x extent must be >= 2");
     if (d.n[2] > 1 && d.n[1] < 2) return fail(SCARL_E_INVALID, "a 3D field needs n[1] >= 2");
     if (d.n[1] < 2) return fail(SCARL_E_UNSUPPORTED, "1D fields are not supported (n[1] = 1)");
@@ -514,8 +513,10 @@ static int build_field(scarl_batch* b, FieldHost& fh) {
     memset(&f, 0, sizeof f);
     for (int a = 0; a < 3; ++a) f.n[a] = d.n[a];
     f.h1 = d.n[0] / 2 + 1;
-    f.wp = d.n[0] / 2;
-    f.nyq = 1;
+    // even n1: rows of n1/2 plus the Nyquist plane; odd n1: rows of h1 (no Nyquist column)
+    const bool odd = (d.n[0] & 1) != 0;
+    f.wp = odd ? f.h1 : d.n[0] / 2;
+    f.nyq = odd ? 0 : 1;
     const int dim = d.n[2] > 1 ? 3 : 2;
     f.first_axis = (dim == 3) ? 2 : 1;
     f.src_mode = SRC_DEVICE_NOISE;
@@ -533,7 +534,8 @@ static int build_field(scarl_batch* b, FieldHost& fh) {
     if (kz < 0) return -kz;
     const int ky = b->ps.get(d.n[1], LAYOUT_PF, 0);
     if (ky < 0) return -ky;
-    const int kx = b->ps.get(d.n[0] / 2, LAYOUT_BF, 1);
+    // x pass: half-length C2R (even n1, with its pre-twiddle table) or a full-length C2C (odd n1)
+    const int kx = odd ? b->ps.get(d.n[0], LAYOUT_BF, 0) : b->ps.get(d.n[0] / 2, LAYOUT_BF, 1);
     if (kx < 0) return -kx;
     f.plan_z = kz;
     f.plan_y = ky;
@@ -786,7 +788,8 @@ static int enqueue_group(scarl_batch* b, const scarl_batch::Group& g, hipStream_
     for (const auto& cl : g.cx) {
         const PlanDev& pl = b->ps.plans[cl.plan];
         SCARL_TRY(timed("h_xpass", [&] {
-            return launch_h_xpass(f64, pl.spec, ecap, b->global, b->d_fields, b->d_tx, cl.t0, cl.t1 - cl.t0,
+            // x-pass plans of odd n1 are plain C2C (no pre-twiddle table)
+            return launch_h_xpass(f64, pl.spec, ecap, b->global, pl.pre_off < 0, b->d_fields, b->d_tx, cl.t0, cl.t1 - cl.t0,
                                   b->ps.d_plans, b->ps.d_pool, b->d_partials, (size_t)pl.lds_elems * csz, s);
         }));
     }

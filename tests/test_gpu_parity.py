@@ -227,6 +227,21 @@ def test_hermitian_fp32_device_noise(hip):
     assert rel_l2(gpu, ref) <= FP32_TOL
 
 
+@pytest.mark.parametrize("n", [(23, 12, 10), (41, 41, 41), (81, 30, 17), (5, 6, 1), (45, 33, 1)])
+def test_hermitian_odd_x_extent(hip, n):
+    """Odd x extents (the reference geometry's L = 1 + ceiling(...) is usually odd): rows of
+    h1 = (n1+1)/2, no Nyquist column, x pass as a full-length C2C of the Hermitian extension."""
+    from scarl_amd import generator
+    _say(f"hermitian odd n={n}")
+    L = (5.0, 4.0, 3.0)
+    spec = generator.FieldSpec(n=n, model=3, corrL=L, seed=13, margi=2, avg=1.0, std_dev=0.4)
+    W = O.noise_hermitian(n, 13, 0)
+    ref = O.gen_mode_h(n, (1, 1, 1), 3, L, W=W, margi=2, avg=1.0, std=0.4)
+    assert rel_l2(generator.gen_field_h(spec, 64, W), ref) <= FP64_TOL
+    refd = O.gen_mode_h(n, (1, 1, 1), 3, L, seed=13, margi=2, avg=1.0, std=0.4)
+    assert rel_l2(generator.gen_field_h(spec, 32), refd) <= FP32_TOL
+
+
 # ---- batches -----------------------------------------------------------------------------
 def test_ensemble_batch_mixed_sizes(hip):
     from scarl_amd import generator
@@ -290,14 +305,14 @@ def test_pipelined_groups_match_single_fields(hip):
                 np.testing.assert_array_equal(mom[i], alone_mom[i])
 
 
-def _blend_reference(gdim, starts, exts, spec):
+def _blend_reference(gdim, starts, exts, spec, dx=(1, 1, 1)):
     fields, st, ws = [], [], []
     sid = 0
     for k in range(len(starts[2])):
         for j in range(len(starts[1])):
             for i in range(len(starts[0])):
                 n = (exts[0][i], exts[1][j], exts[2][k])
-                f = O.gen_mode_h(n, (1, 1, 1), spec.model, spec.corrL, spec.nu, seed=spec.seed, stream=sid)
+                f = O.gen_mode_h(n, dx, spec.model, spec.corrL, spec.nu, seed=spec.seed, stream=sid)
                 pos = (i, j, k)
                 w = []
                 for a in range(3):
@@ -569,3 +584,19 @@ def test_config5_ensemble_256(hip):
         assert rel_l2(outs[i], ref) <= FP32_TOL
         rc = O.lag_covariance(outs[i].astype(np.float64), mom[i, 3], 33)
         np.testing.assert_allclose(cov[i], rc, rtol=0, atol=1e-10 * np.abs(rc).max())
+
+
+def test_deck_geometry_blend(hip):
+    """test/3_TEST sample 1 (x in [0, 10], corrL 1, overlap 1, 5 points per corrL, Gaussian,
+    lognormal 1/1, seed 1) on a 2x2x2 topology: the eight ranks' subdomains (23^3, overlap 5,
+    spacing 0.25) blended in one FP64 batch equal the oracle's gather-sum."""
+    from scarl_amd import geometry, generator
+    g = geometry.sample_geometry((0, 0, 0), (10, 10, 10), (1, 1, 1), (1, 1, 1), (5, 5, 5), (2, 2, 2))
+    with geometry.blend_batch(g, (1, 1, 1), 1, 1, margiFirst=2, avg=1.0, std_dev=1.0, precision=64) as b:
+        b.run()
+        Gf = b.output(-1)
+    gdim, starts, exts = geometry.blend_layout(g)
+    spec = generator.FieldSpec(n=(2, 2, 2), model=1, corrL=(1.0, 1.0, 1.0), seed=2, margi=2, avg=1.0, std_dev=1.0)
+    ref = _blend_reference(gdim, starts, exts, spec, dx=(0.25, 0.25, 0.25))
+    assert Gf.shape == (41, 41, 41)
+    assert rel_l2(Gf, ref) <= FP64_TOL
