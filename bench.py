@@ -258,6 +258,14 @@ def main():
     A.check(lib.scarl_hip_batch_sync(None))
     dtr = (time.perf_counter() - t0) / ks
     hip.hipFree(dptr)
+    # the same call with the host output array of the Fortran interface: PCIe-inclusive rate
+    from scarl_amd import fftw_ScaRL
+    hout = np.empty((256, 256, 256), np.float64, order="F")
+    fftw_ScaRL.gen_std_gauss_FFT(hout, (256, 256, 256), tuple(xr), tuple(cl), 1, 7, 0)
+    t0 = time.perf_counter()
+    for _ in range(3):
+        fftw_ScaRL.gen_std_gauss_FFT(hout, (256, 256, 256), tuple(xr), tuple(cl), 1, 7, 0)
+    dth = (time.perf_counter() - t0) / 3
 
     # ---- CPU baseline (oracle, FP64) and FP32 error on shared noise ----------------------
     cpu = None
@@ -304,6 +312,9 @@ def main():
         "dropin_redft01": {"value": round(256 ** 3 / dtr / 1e9, 3), "unit": "Gpoints/s",
                            "ms_per_field": round(dtr * 1e3, 4),
                            "workload": "gen_std_gauss_FFT 256^3 FP64 + lognormal, device phi, resident"},
+        "dropin_redft01_host_array": {"value": round(256 ** 3 / dth / 1e9, 3), "unit": "Gpoints/s",
+                                      "ms_per_field": round(dth * 1e3, 4),
+                                      "workload": "the same call into a host array (PCIe copy included)"},
         "fp32_vs_fp64": err,
         "moments_field0": {"mean": float(moments[0, 3]), "std": float(moments[0, 4])},
         "cpu_baseline": cpu,
