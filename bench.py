@@ -213,13 +213,44 @@ def main():
     step_frac = step_bytes / dt / 1e9 / HBM_PEAK_GBS
     log(f"bench: passes {json.dumps(passes)}")
 
-    # keep field 0 for the error check
-    f32_field0 = batch.output(0) if not args.no_cpu else None
+    # keep field 0 for the error checks
+    f32_field0 = batch.output(0)
     moments = batch.moments()
     batch.close()
     if args.probe_only:
         print(json.dumps({"copy_probe_GBps": copy_gbs, "value": value}), flush=True)
         return
+
+    # ---- the same batch with the x pass as split-precision bf16 MFMA radix-16 stages ----------
+    log("bench: batched fp32, MFMA radix-16 x pass")
+    Gm.set_tuning(A.TUNE_MFMA_DFT16, 1)
+    try:
+        mb = Gm.Batch.ensemble(specs, A.PREC_FP32)
+    finally:
+        Gm.set_tuning(A.TUNE_MFMA_DFT16, 0)
+    for _ in range(max(args.warmup, 1)):
+        mb.run(use_graph=True)
+    mb.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        mb.run(use_graph=True)
+    mb.sync()
+    dtm = (time.perf_counter() - t0) / args.steps
+    Gm.timing(True)
+    Gm.timing_reset()
+    for _ in range(nruns):
+        mb.run(use_graph=False)
+    mrep = Gm.timing_report()
+    Gm.timing(False)
+    mx = mrep["h_xpass"][1] / mrep["h_xpass"][0]
+    m0 = mb.output(0)
+    mb.close()
+    mfma = {"value": round(pts / dtm / 1e9, 3), "unit": "Gpoints/s", "ms_per_step": round(dtm * 1e3, 4),
+            "h_xpass_avg_ms": mx, "h_xpass_GBps": pb["h_xpass"] * B / (mrep["h_xpass"][0] / nruns) / (mx * 1e-3) / 1e9,
+            "rel_l2_vs_fp32_stockham": float(np.linalg.norm(m0.astype(np.float64) - f32_field0) /
+                                             np.linalg.norm(f32_field0.astype(np.float64))),
+            "scheme": "x pass radix-16 stages as [Re -Im; Im Re] bf16 MFMA products, three-term split, "
+                      "six products, FP32 accumulation"}
 
     # ---- plain FP64, one field per launch ------------------------------------------------
     log("bench: plain fp64, one field per launch")
@@ -282,7 +313,8 @@ def main():
                "sample": f"one {e}^3 lognormal von Karman field (same spec and Philox noise as field 0), "
                          f"FP64 oracle C port, {tc:.2f} s"}
         err = {"rel_l2": float(np.linalg.norm(f32_field0 - ref) / np.linalg.norm(ref)),
-               "max_abs": float(np.abs(f32_field0.astype(np.float64) - ref).max())}
+               "max_abs": float(np.abs(f32_field0.astype(np.float64) - ref).max()),
+               "mfma_rel_l2": float(np.linalg.norm(m0 - ref) / np.linalg.norm(ref))}
 
     out = {
         "metric": METRIC,
@@ -316,6 +348,7 @@ def main():
                                       "ms_per_field": round(dth * 1e3, 4),
                                       "workload": "the same call into a host array (PCIe copy included)"},
         "fp32_vs_fp64": err,
+        "mfma_variant": mfma,
         "moments_field0": {"mean": float(moments[0, 3]), "std": float(moments[0, 4])},
         "cpu_baseline": cpu,
     }

@@ -44,8 +44,11 @@ int scarl_hip_timing_reset(void);
 /* the library's HIP stream (hipStream_t), for callers that order their own work after it */
 int scarl_hip_get_stream(void** stream);
 /* plan tuning for batches created afterwards (no counterpart in the reference; the autotuner's
- * knob).  SCARL_TUNE_FP32_ELEMS: complex elements a thread holds per FFT stage, 16 or 32. */
-enum { SCARL_TUNE_FP32_ELEMS = 1 };
+ * knob).  SCARL_TUNE_FP32_ELEMS: complex elements a thread holds per FFT stage, 16 or 32.
+ * SCARL_TUNE_MFMA_DFT16: 1 runs the FP32 x pass of 512-wide fields (two radix-16 stages) as
+ * split-precision bf16 matrix products on the MFMA units (three bf16 terms per operand, six
+ * products, FP32 accumulation); 0 (default) keeps the FP32 Stockham butterflies. */
+enum { SCARL_TUNE_FP32_ELEMS = 1, SCARL_TUNE_MFMA_DFT16 = 2 };
 int scarl_hip_set_tuning(int32_t key, int32_t value);
 
 /* ---- fftw_ScaRL drop-in ------------------------------------------------------------------ */

@@ -115,6 +115,10 @@ struct Tile {
     X(17, 8, LAYOUT_BF, 256, 16, 16, 32)    \
     X(18, 8, LAYOUT_BF, 128, 16, 8, 16)
 
+// The FP32 x pass of a 512-wide field (N = 256 = 16 x 16) on the matrix cores: split-precision
+// bf16 MFMA radix-16 stages (scarl_mfma.h), selected by SCARL_TUNE_MFMA_DFT16.
+constexpr int kSpecMfma256 = 64;
+
 // Compensated (Neumaier) partial sums of x and x^2 for one tile.
 struct MomPartial {
     double s, c, s2, c2;

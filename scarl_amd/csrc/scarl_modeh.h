@@ -8,6 +8,7 @@
 // For a 2D field (n3 = 1) the y pass generates.
 #pragma once
 #include "scarl_fft.h"
+#include "scarl_mfma.h"
 
 namespace scarl {
 
@@ -349,6 +350,10 @@ struct ModeHDispatch {
         if (full) {
             if (ecap == E2) xpass<GenericFft<E2>, true>(blend, fields, tiles, tile0, ntiles, plans, p, partials, lds, s);
             else xpass<GenericFft<E1>, true>(blend, fields, tiles, tile0, ntiles, plans, p, partials, lds, s);
+            return hipGetLastError();
+        }
+        if (spec == kSpecMfma256) {
+            if constexpr (PREC == 4) xpass<MfmaFft256>(blend, fields, tiles, tile0, ntiles, plans, p, partials, lds, s);
             return hipGetLastError();
         }
         switch (spec) {

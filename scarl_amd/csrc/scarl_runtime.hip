@@ -126,6 +126,7 @@ struct Context {
     int margi = 0;
     double m_avg = 0.0, m_std = 1.0;
     int fp32_ecap = 16;  // register budget (complex elements per thread) of new FP32 plans
+    bool mfma_dft16 = false;  // FP32 256-point x passes on the matrix cores (new batches)
 };
 
 static Context g_ctx;
@@ -229,6 +230,7 @@ struct PlanSet {
     PlanDev* d_plans = nullptr;
     void* d_pool = nullptr;
     bool fp64 = false;
+    bool mfma = false;   // plan the FP32 256-point x pass on the matrix cores
     int ecap = kEcap64;  // register budget of every plan of the set (matches a kernel instantiation)
     size_t max_lds = 0;
 
@@ -294,6 +296,8 @@ struct PlanSet {
         SCARL_FIXED_SPECS(SCARL_MATCH)
 #undef SCARL_MATCH
         pl.spec = spec;
+        if (mfma && !fp64 && layout == LAYOUT_BF && kind == 1 && N == 256 && sr0 == 16 && sr1 == 16 && stpp == 16)
+            pl.spec = kSpecMfma256;  // same radices, tile and twiddle tables: only the kernel differs
         if (spec) {
             pl.nst = sr1 > 1 ? 2 : 1;
             pl.R[0] = sr0;
@@ -999,6 +1003,11 @@ int scarl_hip_set_tuning(int32_t key, int32_t value) {
         g_ctx.fp32_ecap = value;
         return 0;
     }
+    if (key == SCARL_TUNE_MFMA_DFT16) {
+        if (value != 0 && value != 1) return fail(SCARL_E_INVALID, "SCARL_TUNE_MFMA_DFT16 takes 0 or 1");
+        g_ctx.mfma_dft16 = value != 0;
+        return 0;
+    }
     return fail(SCARL_E_INVALID, "unknown tuning key %d", key);
 }
 
@@ -1446,6 +1455,7 @@ int scarl_hip_batch_create(const scarl_field_desc* fields, int32_t count, int32_
     b->fp64 = precision == SCARL_PREC_FP64;
     b->ps.fp64 = b->fp64;
     b->ps.ecap = b->fp64 ? kEcap64 : g_ctx.fp32_ecap;
+    b->ps.mfma = g_ctx.mfma_dft16;
     b->fields.resize(count);
     std::vector<double> slot_count(count);
     for (int i = 0; i < count; ++i) {
@@ -1506,6 +1516,7 @@ int scarl_hip_blend_create(const int32_t gdim[3], const int32_t nb[3], const int
     b->fp64 = precision == SCARL_PREC_FP64;
     b->ps.fp64 = b->fp64;
     b->ps.ecap = b->fp64 ? kEcap64 : g_ctx.fp32_ecap;
+    b->ps.mfma = g_ctx.mfma_dft16;
     b->global = true;
     for (int a = 0; a < 3; ++a) b->gdim[a] = gdim[a];
     const int count = nb[0] * nb[1] * nb[2];

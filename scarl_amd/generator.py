@@ -230,3 +230,10 @@ def seeded_tiling(gdim, nb, overlap, choices, seed):
         starts.append(s)
         exts.append([int(v) for v in e])
     return starts, exts
+
+
+def set_tuning(key, value):
+    """Plan tuning for batches created afterwards: A.TUNE_FP32_ELEMS (16 or 32 complex elements
+    per thread) or A.TUNE_MFMA_DFT16 (1: the FP32 x pass of 512-wide fields as split-precision
+    bf16 MFMA radix-16 stages; 0: FP32 Stockham butterflies)."""
+    A.call("scarl_hip_set_tuning", int(key), int(value))

@@ -600,3 +600,31 @@ def test_deck_geometry_blend(hip):
     ref = _blend_reference(gdim, starts, exts, spec, dx=(0.25, 0.25, 0.25))
     assert Gf.shape == (41, 41, 41)
     assert rel_l2(Gf, ref) <= FP64_TOL
+
+
+@pytest.mark.parametrize("n", [(512, 16, 8), (512, 20, 1)])
+def test_mfma_dft16_x_pass(hip, n):
+    """The FP32 x pass of a 512-wide field as split-precision bf16 MFMA radix-16 stages: against
+    the FP64 oracle (FP32 bound) and against the FP32 Stockham pass, plain and blended."""
+    from scarl_amd import _capi as A, generator
+    _say(f"mfma x pass n={n}")
+    spec = generator.FieldSpec(n=n, model=2, corrL=(24.0, 6.0, 4.0), nu=0.3, seed=3, margi=2, avg=1.0, std_dev=0.3)
+    W = O.noise_hermitian(n, 3, 0).astype(np.complex64)
+    ref = O.gen_mode_h(n, (1, 1, 1), 2, (24.0, 6.0, 4.0), 0.3, W=W.astype(np.complex128), margi=2, avg=1.0, std=0.3)
+    stock = generator.gen_field_h(spec, 32, W)
+    gdim = (1000, n[1], n[2])
+    try:
+        generator.set_tuning(A.TUNE_MFMA_DFT16, 1)
+        mf = generator.gen_field_h(spec, 32, W)
+        with generator.Batch.blend(gdim, [[0, 488], [0], [0]], [[512, 512], [n[1]], [n[2]]], spec, 32) as b:
+            b.run()
+            bm = b.output(-1)
+    finally:
+        generator.set_tuning(A.TUNE_MFMA_DFT16, 0)
+    with generator.Batch.blend(gdim, [[0, 488], [0], [0]], [[512, 512], [n[1]], [n[2]]], spec, 32) as b:
+        b.run()
+        bs = b.output(-1)
+    e_ref, e_st = rel_l2(mf, ref), rel_l2(mf, stock)
+    _say(f"  mfma vs fp64 oracle {e_ref:.2e} (stockham {rel_l2(stock, ref):.2e}), vs stockham {e_st:.2e}")
+    assert e_ref <= FP32_TOL and e_st <= FP32_TOL
+    assert rel_l2(bm, bs) <= FP32_TOL
