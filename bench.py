@@ -247,6 +247,9 @@ def main():
     mb.close()
     mfma = {"value": round(pts / dtm / 1e9, 3), "unit": "Gpoints/s", "ms_per_step": round(dtm * 1e3, 4),
             "h_xpass_avg_ms": mx, "h_xpass_GBps": pb["h_xpass"] * B / (mrep["h_xpass"][0] / nruns) / (mx * 1e-3) / 1e9,
+            # per pencil: 2 stages x 2 output tiles (Re, Im) x 6 split products of a 16x16x32 bf16 MFMA
+            "mfma_TFLOPs": (e * e) * 2 * 2 * 6 * (16 * 16 * 32 * 2) / (mx * 1e-3) / 1e12,
+            "mfma_frac_of_bf16_dense_peak": (e * e) * 2 * 2 * 6 * (16 * 16 * 32 * 2) / (mx * 1e-3) / 2.5e15,
             "rel_l2_vs_fp32_stockham": float(np.linalg.norm(m0.astype(np.float64) - f32_field0) /
                                              np.linalg.norm(f32_field0.astype(np.float64))),
             "scheme": "x pass radix-16 stages as [Re -Im; Im Re] bf16 MFMA products, three-term split, "

@@ -14,6 +14,7 @@
 #include <map>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/scarl_hip.h"
@@ -127,6 +128,8 @@ struct Context {
     double m_avg = 0.0, m_std = 1.0;
     int fp32_ecap = 16;  // register budget (complex elements per thread) of new FP32 plans
     bool mfma_dft16 = false;  // FP32 256-point x passes on the matrix cores (new batches)
+    void* stage[2] = {nullptr, nullptr};         // pinned staging buffers for caller arrays
+    hipEvent_t stage_ev[2] = {nullptr, nullptr};  // last DMA that used each staging buffer
 };
 
 static Context g_ctx;
@@ -154,6 +157,85 @@ static int ensure_init() {
     HIP_TRY(hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking));
     HIP_TRY(hipStreamCreateWithFlags(&g_ctx.gen_stream, hipStreamNonBlocking));
     g_ctx.ready = true;
+    return 0;
+}
+
+// Copies between caller-owned (pageable) host arrays and device memory go through two pinned
+// staging buffers: the DMA of one chunk overlaps the host copy of the other (up to 8 threads),
+// about 4x the throughput of a direct pageable copy.  Both calls are ordered on the library stream
+// and return when the host side is done (copy_to_device: the data may still be in flight to the
+// device, ahead of any later work on the stream).
+constexpr size_t kStageBytes = size_t(32) << 20;
+constexpr size_t kStageMin = size_t(4) << 20;
+
+static int stage_ready() {
+    if (g_ctx.stage[0]) return 0;
+    for (int i = 0; i < 2; ++i) {
+        HIP_TRY(hipHostMalloc(&g_ctx.stage[i], kStageBytes, hipHostMallocDefault));
+        HIP_TRY(hipEventCreateWithFlags(&g_ctx.stage_ev[i], hipEventDisableTiming));
+    }
+    return 0;
+}
+
+static void host_copy(void* dst, const void* src, size_t bytes) {
+    const int nt = bytes >= (size_t(8) << 20) ? 8 : 1;
+    if (nt == 1) {
+        memcpy(dst, src, bytes);
+        return;
+    }
+    std::vector<std::thread> th;
+    const size_t part = (bytes + nt - 1) / nt;
+    for (int t = 0; t < nt; ++t) {
+        const size_t o = (size_t)t * part;
+        if (o >= bytes) break;
+        const size_t n = std::min(part, bytes - o);
+        th.emplace_back([=] { memcpy(static_cast<char*>(dst) + o, static_cast<const char*>(src) + o, n); });
+    }
+    for (auto& x : th) x.join();
+}
+
+static int copy_to_host(void* h, const void* d, size_t bytes) {
+    if (bytes < kStageMin) {
+        HIP_TRY(hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, g_ctx.stream));
+        HIP_TRY(hipStreamSynchronize(g_ctx.stream));
+        return 0;
+    }
+    SCARL_TRY(stage_ready());
+    const size_t nch = (bytes + kStageBytes - 1) / kStageBytes;
+    auto enqueue = [&](size_t c) -> int {
+        const size_t o = c * kStageBytes, n = std::min(kStageBytes, bytes - o);
+        HIP_TRY(hipMemcpyAsync(g_ctx.stage[c & 1], static_cast<const char*>(d) + o, n, hipMemcpyDeviceToHost,
+                               g_ctx.stream));
+        HIP_TRY(hipEventRecord(g_ctx.stage_ev[c & 1], g_ctx.stream));
+        return 0;
+    };
+    SCARL_TRY(enqueue(0));
+    for (size_t c = 0; c < nch; ++c) {
+        // the other buffer's host copy finished in the previous iteration
+        if (c + 1 < nch) SCARL_TRY(enqueue(c + 1));
+        HIP_TRY(hipEventSynchronize(g_ctx.stage_ev[c & 1]));
+        const size_t o = c * kStageBytes;
+        host_copy(static_cast<char*>(h) + o, g_ctx.stage[c & 1], std::min(kStageBytes, bytes - o));
+    }
+    return 0;
+}
+
+static int copy_to_device(void* d, const void* h, size_t bytes) {
+    if (bytes < kStageMin) {
+        HIP_TRY(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, g_ctx.stream));
+        HIP_TRY(hipStreamSynchronize(g_ctx.stream));
+        return 0;
+    }
+    SCARL_TRY(stage_ready());
+    const size_t nch = (bytes + kStageBytes - 1) / kStageBytes;
+    for (size_t c = 0; c < nch; ++c) {
+        const size_t o = c * kStageBytes, n = std::min(kStageBytes, bytes - o);
+        HIP_TRY(hipEventSynchronize(g_ctx.stage_ev[c & 1]));  // the buffer's previous DMA is done
+        host_copy(g_ctx.stage[c & 1], static_cast<const char*>(h) + o, n);
+        HIP_TRY(hipMemcpyAsync(static_cast<char*>(d) + o, g_ctx.stage[c & 1], n, hipMemcpyHostToDevice,
+                               g_ctx.stream));
+        HIP_TRY(hipEventRecord(g_ctx.stage_ev[c & 1], g_ctx.stream));
+    }
     return 0;
 }
 
@@ -949,6 +1031,15 @@ int scarl_hip_finalize(void) {
     g_ctx.pending.clear();
     g_r.ps.reset();
     memset(g_r.n, 0, sizeof g_r.n);
+    for (int i = 0; i < 2; ++i) {
+        if (g_ctx.stage_ev[i]) {
+            hipEventSynchronize(g_ctx.stage_ev[i]);
+            hipEventDestroy(g_ctx.stage_ev[i]);
+        }
+        if (g_ctx.stage[i]) hipHostFree(g_ctx.stage[i]);
+        g_ctx.stage[i] = nullptr;
+        g_ctx.stage_ev[i] = nullptr;
+    }
     hipStreamDestroy(g_ctx.stream);
     hipStreamDestroy(g_ctx.gen_stream);
     g_ctx.stream = nullptr;
@@ -1032,12 +1123,11 @@ static int gen_r_host(double* h_out, const int32_t n[3], const int32_t m[3], con
     SCARL_TRY(scratch_get("r_out", (size_t)std::max<int64_t>(1, N) * 8, &d_out));
     if (h_phi) {
         SCARL_TRY(scratch_get("r_phi", (size_t)std::max<int64_t>(1, P) * 8, &d_phi));
-        if (P > 0) HIP_TRY(hipMemcpyAsync(d_phi, h_phi, (size_t)P * 8, hipMemcpyHostToDevice, g_ctx.stream));
+        if (P > 0) SCARL_TRY(copy_to_device(d_phi, h_phi, (size_t)P * 8));
     }
     const uint64_t key = (uint64_t)(int64_t)((int64_t)seed + rank);
     SCARL_TRY(mode_r_run((double*)d_out, n, m, xRange, corrL, corrMod, key, (const double*)d_phi, flags));
-    HIP_TRY(hipMemcpyAsync(h_out, d_out, (size_t)N * 8, hipMemcpyDeviceToHost, g_ctx.stream));
-    HIP_TRY(hipStreamSynchronize(g_ctx.stream));
+    SCARL_TRY(copy_to_host(h_out, d_out, (size_t)N * 8));
     return 0;
 }
 
@@ -1081,8 +1171,8 @@ int scarl_hip_r8vec_uniform_01(int32_t n, int32_t* seed, double* h_r) {
         return launch_lcg_uniform(*seed, n, (double*)d_r, (int32_t*)d_s, g_ctx.stream);
     }));
     int32_t last = 0;
-    HIP_TRY(hipMemcpyAsync(h_r, d_r, (size_t)n * 8, hipMemcpyDeviceToHost, g_ctx.stream));
-    HIP_TRY(hipMemcpyAsync(&last, (int32_t*)d_s + (n - 1), 4, hipMemcpyDeviceToHost, g_ctx.stream));
+    SCARL_TRY(copy_to_host(h_r, d_r, (size_t)n * 8));
+    SCARL_TRY(copy_to_host(&last, (int32_t*)d_s + (n - 1), 4));
     HIP_TRY(hipStreamSynchronize(g_ctx.stream));
     *seed = last;
     return 0;
@@ -1131,12 +1221,12 @@ int scarl_hip_apply_unity_partition(double* h_field, const int32_t Np[3], const 
     const int64_t N = (int64_t)Np[0] * Np[1] * Np[2];
     void* d;
     SCARL_TRY(scratch_get("pou", (size_t)N * 8, &d));
-    HIP_TRY(hipMemcpyAsync(d, h_field, (size_t)N * 8, hipMemcpyHostToDevice, g_ctx.stream));
+    SCARL_TRY(copy_to_device(d, h_field, (size_t)N * 8));
     SCARL_TRY(timed("pou", [&] {
         return launch_pou((double*)d, Np[0], Np[1], Np[2], make_int3(lo[0], lo[1], lo[2]),
                           make_int3(hi[0], hi[1], hi[2]), g_ctx.stream);
     }));
-    HIP_TRY(hipMemcpyAsync(h_field, d, (size_t)N * 8, hipMemcpyDeviceToHost, g_ctx.stream));
+    SCARL_TRY(copy_to_host(h_field, d, (size_t)N * 8));
     HIP_TRY(hipStreamSynchronize(g_ctx.stream));
     return 0;
 }
@@ -1151,9 +1241,9 @@ int scarl_hip_multivariate_transformation(double avg, double std_dev, int32_t ma
     if (!h_field) return fail(SCARL_E_INVALID, "null field");
     void* d;
     SCARL_TRY(scratch_get("marg", (size_t)n * 8, &d));
-    HIP_TRY(hipMemcpyAsync(d, h_field, (size_t)n * 8, hipMemcpyHostToDevice, g_ctx.stream));
+    SCARL_TRY(copy_to_device(d, h_field, (size_t)n * 8));
     SCARL_TRY(timed("marginal", [&] { return launch_marginal((double*)d, n, margiFirst, mu, sd, g_ctx.stream); }));
-    HIP_TRY(hipMemcpyAsync(h_field, d, (size_t)n * 8, hipMemcpyDeviceToHost, g_ctx.stream));
+    SCARL_TRY(copy_to_host(h_field, d, (size_t)n * 8));
     HIP_TRY(hipStreamSynchronize(g_ctx.stream));
     return 0;
 }
@@ -1168,10 +1258,10 @@ int scarl_hip_moments(const double* h_rf, int64_t n, int32_t nmc, double* evnt_a
     void *d, *dp;
     SCARL_TRY(scratch_get("mom_in", (size_t)(n * nmc) * 8, &d));
     SCARL_TRY(scratch_get("mom_part", (size_t)nblocks * nmc * sizeof(MomPartial), &dp));
-    HIP_TRY(hipMemcpyAsync(d, h_rf, (size_t)(n * nmc) * 8, hipMemcpyHostToDevice, g_ctx.stream));
+    SCARL_TRY(copy_to_device(d, h_rf, (size_t)(n * nmc) * 8));
     SCARL_TRY(timed("sums", [&] { return launch_sums((const double*)d, n, nmc, nblocks, (MomPartial*)dp, g_ctx.stream); }));
     std::vector<MomPartial> h((size_t)nblocks * nmc);
-    HIP_TRY(hipMemcpyAsync(h.data(), dp, h.size() * sizeof(MomPartial), hipMemcpyDeviceToHost, g_ctx.stream));
+    SCARL_TRY(copy_to_host(h.data(), dp, h.size() * sizeof(MomPartial)));
     HIP_TRY(hipStreamSynchronize(g_ctx.stream));
     auto nadd_h = [](double& s, double& c, double v) {
         const double t = s + v;
@@ -1232,7 +1322,7 @@ static int run_lag_jobs(std::vector<LagJob>& jobs, bool fp64, int nlag_max, doub
     }
     SCARL_TRY(scratch_get("lag_jobs", jobs.size() * sizeof(LagJob), &d_jobs));
     SCARL_TRY(scratch_get("lag_blocks", std::max<size_t>(1, blocks.size()) * sizeof(LagBlock), &d_blocks));
-    HIP_TRY(hipMemcpyAsync(d_jobs, jobs.data(), jobs.size() * sizeof(LagJob), hipMemcpyHostToDevice, g_ctx.stream));
+    SCARL_TRY(copy_to_device(d_jobs, jobs.data(), jobs.size() * sizeof(LagJob)));
     if (!blocks.empty())
         HIP_TRY(hipMemcpyAsync(d_blocks, blocks.data(), blocks.size() * sizeof(LagBlock), hipMemcpyHostToDevice,
                                g_ctx.stream));
@@ -1283,7 +1373,7 @@ int scarl_hip_rebuild_sk_fft(const double* h_field, const int32_t xNStep[3], int
     void *d_x, *d_out;
     SCARL_TRY(scratch_get("stat_field", (size_t)tot * 8, &d_x));
     SCARL_TRY(scratch_get("stat_sk", (size_t)nd * 8, &d_out));
-    HIP_TRY(hipMemcpyAsync(d_x, h_field, (size_t)tot * 8, hipMemcpyHostToDevice, g_ctx.stream));
+    SCARL_TRY(copy_to_device(d_x, h_field, (size_t)tot * 8));
     std::vector<LagJob> jobs(1, lag_job(d_x, n, dir - 1, nd, 1, nDim == 3, avg, nullptr, 1.0 / stdDev));
     SCARL_TRY(run_lag_jobs(jobs, true, nd, (double*)d_out));
     HIP_TRY(hipMemcpy(h_sk_part, d_out, (size_t)nd * 8, hipMemcpyDeviceToHost));
@@ -1320,7 +1410,7 @@ int scarl_hip_lag_covariance(const double* h_field, const int32_t n[3], double m
     void *d_x, *d_out;
     SCARL_TRY(scratch_get("stat_field", (size_t)tot * 8, &d_x));
     SCARL_TRY(scratch_get("stat_cov", (size_t)3 * nlag * 8, &d_out));
-    HIP_TRY(hipMemcpyAsync(d_x, h_field, (size_t)tot * 8, hipMemcpyHostToDevice, g_ctx.stream));
+    SCARL_TRY(copy_to_device(d_x, h_field, (size_t)tot * 8));
     std::vector<LagJob> jobs;
     for (int a = 0; a < 3; ++a) jobs.push_back(lag_job(d_x, n, a, nlag, 0, 0, mean, nullptr, 1.0));
     SCARL_TRY(run_lag_jobs(jobs, true, nlag, (double*)d_out));
@@ -1379,13 +1469,13 @@ int scarl_hip_normalize_field(double* h_field, const int32_t Np[3], const int32_
     void *d_x, *d_part;
     SCARL_TRY(scratch_get("norm_field", (size_t)tot * 8, &d_x));
     SCARL_TRY(scratch_get("norm_part", (size_t)nblocks * sizeof(MomPartial), &d_part));
-    HIP_TRY(hipMemcpyAsync(d_x, h_field, (size_t)tot * 8, hipMemcpyHostToDevice, g_ctx.stream));
+    SCARL_TRY(copy_to_device(d_x, h_field, (size_t)tot * 8));
     std::vector<MomPartial> h(nblocks);
     auto box_sum = [&](double centre, bool squares, double* out) -> int {
         SCARL_TRY(timed("box_sums", [&] {
             return launch_box_sums((const double*)d_x, n, box, centre, (MomPartial*)d_part, nblocks, g_ctx.stream);
         }));
-        HIP_TRY(hipMemcpyAsync(h.data(), d_part, h.size() * sizeof(MomPartial), hipMemcpyDeviceToHost, g_ctx.stream));
+        SCARL_TRY(copy_to_host(h.data(), d_part, h.size() * sizeof(MomPartial)));
         HIP_TRY(hipStreamSynchronize(g_ctx.stream));
         double s = 0.0, c = 0.0;
         for (const MomPartial& m : h) {
@@ -1405,7 +1495,7 @@ int scarl_hip_normalize_field(double* h_field, const int32_t Np[3], const int32_
     const double sd = sqrt(S2 / xnt);
     if (!(sd > 0.0)) return fail(SCARL_E_INVALID, "the field is constant over the box: std_dev = 0");
     SCARL_TRY(timed("normalize", [&] { return launch_normalize((double*)d_x, tot, avg, sd, g_ctx.stream); }));
-    HIP_TRY(hipMemcpyAsync(h_field, d_x, (size_t)tot * 8, hipMemcpyDeviceToHost, g_ctx.stream));
+    SCARL_TRY(copy_to_host(h_field, d_x, (size_t)tot * 8));
     HIP_TRY(hipStreamSynchronize(g_ctx.stream));
     if (avg_out) *avg_out = avg;
     if (std_out) *std_out = sd;
@@ -1420,14 +1510,14 @@ static int run_fp64_map(int which, int32_t model, double nu, const double* h_x, 
     void *dx, *dy;
     SCARL_TRY(scratch_get("map_x", (size_t)n * 8, &dx));
     SCARL_TRY(scratch_get("map_y", (size_t)n * 8, &dy));
-    HIP_TRY(hipMemcpyAsync(dx, h_x, (size_t)n * 8, hipMemcpyHostToDevice, g_ctx.stream));
+    SCARL_TRY(copy_to_device(dx, h_x, (size_t)n * 8));
     if (which == 0)
         SCARL_TRY(timed("kv", [&] { return launch_kv(nu, (const double*)dx, n, (double*)dy, g_ctx.stream); }));
     else
         SCARL_TRY(timed("target_corr", [&] {
             return launch_target_corr(model, nu, (const double*)dx, n, (double*)dy, g_ctx.stream);
         }));
-    HIP_TRY(hipMemcpyAsync(h_out, dy, (size_t)n * 8, hipMemcpyDeviceToHost, g_ctx.stream));
+    SCARL_TRY(copy_to_host(h_out, dy, (size_t)n * 8));
     HIP_TRY(hipStreamSynchronize(g_ctx.stream));
     return 0;
 }
@@ -1571,7 +1661,7 @@ int scarl_hip_batch_set_noise(scarl_batch* b, int32_t field, const void* h_noise
     const size_t bytes = (size_t)fh.f.h1 * fh.f.n[1] * fh.f.n[2] * 2 * (b->fp64 ? 8 : 4);
     if (h_noise) {
         if (!fh.noise_buf) SCARL_TRY(dmalloc(&fh.noise_buf, bytes));
-        HIP_TRY(hipMemcpyAsync(fh.noise_buf, h_noise, bytes, hipMemcpyHostToDevice, g_ctx.stream));
+        SCARL_TRY(copy_to_device(fh.noise_buf, h_noise, bytes));
         fh.f.W = fh.noise_buf;
         fh.f.src_mode = SRC_HOST_NOISE;
     } else {
@@ -1638,8 +1728,7 @@ int scarl_hip_batch_copy_output(scarl_batch* b, int32_t field, void* h_dst, int6
     const int64_t need = dims[0] * dims[1] * dims[2] * (b->fp64 ? 8 : 4);
     if (!h_dst || nbytes < need) return fail(SCARL_E_INVALID, "destination holds %lld bytes, need %lld",
                                              (long long)nbytes, (long long)need);
-    HIP_TRY(hipMemcpyAsync(h_dst, p, (size_t)need, hipMemcpyDeviceToHost, g_ctx.stream));
-    HIP_TRY(hipStreamSynchronize(g_ctx.stream));
+    SCARL_TRY(copy_to_host(h_dst, p, (size_t)need));
     return 0;
 }
 
@@ -1647,8 +1736,7 @@ int scarl_hip_batch_moments(scarl_batch* b, double* h_out, int32_t max_slots, in
     SCARL_TRY(ensure_init());
     if (!b || !h_out || !count) return fail(SCARL_E_INVALID, "null argument");
     const int n = std::min(max_slots, b->nslots);
-    HIP_TRY(hipMemcpyAsync(h_out, b->d_slot_out, (size_t)n * 5 * sizeof(double), hipMemcpyDeviceToHost,
-                           g_ctx.stream));
+    SCARL_TRY(copy_to_host(h_out, b->d_slot_out, (size_t)n * 5 * sizeof(double)));
     HIP_TRY(hipStreamSynchronize(g_ctx.stream));
     *count = b->nslots;
     return 0;
@@ -1742,7 +1830,7 @@ int scarl_hip_build_sk(int32_t corrMod, const double xRange[3], const double cor
     int32_t shape[3];
     SCARL_TRY(build_sk_device(corrMod, xRange, Np, sz_c, seed, &d_sk, shape));
     const size_t bytes = (size_t)shape[0] * shape[1] * shape[2] * 16;
-    HIP_TRY(hipMemcpyAsync(h_sk, d_sk, bytes, hipMemcpyDeviceToHost, g_ctx.stream));
+    SCARL_TRY(copy_to_host(h_sk, d_sk, bytes));
     HIP_TRY(hipStreamSynchronize(g_ctx.stream));
     return 0;
 }
