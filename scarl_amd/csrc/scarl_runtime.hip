@@ -1595,12 +1595,17 @@ int scarl_hip_blend_create(const int32_t gdim[3], const int32_t nb[3], const int
                 if (st[a][p] <= st[a][p - 1]) return fail(SCARL_E_INVALID, "axis %d: starts must increase", a);
                 if (p > 1 && st[a][p] < st[a][p - 2] + ex[a][p - 2])
                     return fail(SCARL_E_INVALID, "axis %d: overlap reaches beyond the neighbour", a);
-                if (2 * ov > ex[a][p] || 2 * ov > ex[a][p - 1])
-                    return fail(SCARL_E_INVALID, "axis %d: overlap %d larger than half a subdomain", a, ov);
+                if (ov > ex[a][p] || ov > ex[a][p - 1])
+                    return fail(SCARL_E_INVALID, "axis %d: overlap %d larger than a subdomain", a, ov);
                 lo[a][p] = ov;
                 hi[a][p - 1] = ov;
             }
         }
+        // the two ramps of a subdomain must not meet (the reference's expansion rule,
+        // SRC_F/main_ScaRL.f90:161-162, guarantees it for its uniform tilings)
+        for (int p = 0; p < nb[a]; ++p)
+            if (lo[a][p] + hi[a][p] > ex[a][p])
+                return fail(SCARL_E_INVALID, "axis %d: the overlaps of subdomain %d exceed its extent", a, p);
     }
     std::unique_ptr<scarl_batch> b(new scarl_batch());
     b->fp64 = precision == SCARL_PREC_FP64;

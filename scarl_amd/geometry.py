@@ -12,7 +12,9 @@ Restates the geometry chain of ``program main_ScaRL`` that feeds every rank's ge
   coordinates, ``xRange``, the overlap length and ``xMaxGlob`` (:455-468);
 * ``seedStart = seedBase + 1 + rank`` (:499-507); a negative seedBase (clock-based seed) is the
   caller's to resolve;
-* ``get_topo_pos`` / ``get_topo_rank``: x-fastest rank order (:747-782).
+* ``get_topo_pos`` / ``get_topo_rank``: x-fastest rank order (:747-782);
+* ``decide_topo_shape`` / ``find_factors``: the subdomain grid for nb_procs ranks when no
+  extra_config_ScaRL.txt gives one (:786-978).
 
 The reference gives each subdomain to its own MPI rank. Here all the subdomains of a sample go
 into one blended batch on one GPU. Subdomain id = rank, so device noise is keyed by
@@ -41,6 +43,65 @@ def get_topo_rank(topo_shape: Sequence[int], topo_pos: Sequence[int]) -> int:
     """SRC_F/main_ScaRL.f90:770-782."""
     return int(topo_shape[0]) * int(topo_shape[1]) * int(topo_pos[2]) + int(topo_shape[0]) * int(topo_pos[1]) + int(
         topo_pos[0])
+
+
+def find_factors(n: int):
+    """Prime factors of n in ascending order (SRC_F/main_ScaRL.f90:959-978)."""
+    d, div, nxt, rest = [], 2, 3, int(n)
+    while rest != 1:
+        while rest % div == 0:
+            d.append(div)
+            rest //= div
+        div, nxt = nxt, nxt + 2
+    return d
+
+
+def decide_topo_shape(nb_procs: int, L, nPointsOvlp, pointsPerBlockIdeal: int = 1, gen_meth=FFT):
+    """SRC_F/main_ScaRL.f90:786-897.
+
+    FFT: grow the block count on the axis with the most points per block until a block holds at
+    most pointsPerBlockIdeal points (1 in the reference, :29) or there are nb_procs blocks. Then,
+    for every rank count np from ceiling(0.75 nb_procs) to nb_procs, spread np's prime factors
+    (largest first) over the axes with the most blocks, and keep the grid with the best
+    volume/surface ratio prod(n) / (2 (n1 n2 + n2 n3 + n3 n1)). The reference compares against
+    vol_surf_factor before setting it; it is read here as 0, so the first grid is always taken.
+    Later grids replace it on ties.
+
+    FFT_MPI: FFTW-MPI slabs along z, and topo_shape = (1, 1, ranks with planes).
+    fftw_mpi_local_size_3d's default block is ceiling(L3 / nb_procs) planes per rank.
+    """
+    L = [int(v) for v in L]
+    if gen_meth == FFT_MPI:
+        block = -(-L[2] // int(nb_procs))
+        return [1, 1, -(-L[2] // block)]
+    nov = [int(v) for v in nPointsOvlp]
+    nBlocks = [1, 1, 1]
+    while True:
+        nPoints = [L[a] + (nBlocks[a] - 1) * nov[a] for a in range(3)]
+        per = [math.ceil(float(nPoints[a]) / float(nBlocks[a])) for a in range(3)]
+        if per[0] * per[1] * per[2] <= pointsPerBlockIdeal:
+            break
+        if nBlocks[0] * nBlocks[1] * nBlocks[2] >= nb_procs:
+            break
+        pos = per.index(max(per))
+        nBlocks[pos] += 1
+    ideal = list(nBlocks)
+    vol_surf = 0.0
+    chosen = [1, 1, 1]
+    np_start = math.ceil(0.75 * nb_procs)  # REAL(4) 0.75 is exact
+    for npr in range(np_start, nb_procs + 1):
+        nb = list(ideal)
+        nf = [1, 1, 1]
+        for fct in reversed(find_factors(npr)):
+            if all(v == 1 for v in nb):
+                break
+            pos = nb.index(max(nb))
+            nf[pos] *= fct
+            nb[pos] = math.ceil(float(nb[pos]) / float(fct))
+        f = float(nf[0] * nf[1] * nf[2]) / float(2 * (nf[0] * nf[1] + nf[1] * nf[2] + nf[2] * nf[0]))
+        if f >= vol_surf:
+            vol_surf, chosen = f, nf
+    return chosen
 
 
 def sample_geometry(xMinGlob, xMaxGlob, corrL, overlap, pointsPerCorrL, topo_shape, gen_meth=FFT):

@@ -1,0 +1,128 @@
+"""The deck-driven program (scarl_amd.main_ScaRL.main) on the reference's decks: HDF5/XDMF files
+written from one batched GPU generation, checked against the reference's file layout
+(write_output.f90) and against the same generation run directly through generator.Batch."""
+import datetime
+import os
+import shutil
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DECKS = os.path.join(ROOT, "tests", "golden", "decks")
+
+pytestmark = pytest.mark.gpu
+
+LABELS = ["Read_input", "Topology", "Generation", "Apply_unit", "Add_overlap", "Normalize", "Multivar_trans",
+          "Write_file", "Total_Wall_time"]
+
+
+def _deck(name, tmp_path, subst=None, config_subst=None):
+    d = tmp_path / name
+    shutil.copytree(os.path.join(DECKS, name), d)
+    for fname, sub in (("input_ScaRL.txt", subst), ("config_ScaRL.txt", config_subst)):
+        if sub:
+            p = d / fname
+            txt = p.read_text()
+            for k, v in sub.items():
+                txt = txt.replace(k, v)
+            p.write_text(txt)
+    return d
+
+
+def _direct_blend(g, s, precision=32):
+    """The same blended generation, straight through generator.Batch."""
+    from scarl_amd import geometry as G
+    with G.blend_batch(g, s.corrL, s.corrMod, s.seedBase, s.margiFirst, s.avg, s.std_dev,
+                       precision=precision) as b:
+        b.run()
+        return b.output(-1)
+
+
+def test_3_test_deck_one_file_one_dataset(hip, tmp_path):
+    from scarl_amd import main_ScaRL, read_input as R, write_output as W
+    d = _deck("3_TEST", tmp_path)
+    print("case: 3_TEST deck, 8 ranks, one file / one dataset", flush=True)
+    (res,) = main_ScaRL.main(str(d), nb_procs=8)
+    g = res["geometry"]
+    # xStep 0.25, L = 41, Np_ovlp = 5; 8 ranks -> 2x2x2 subdomains of 23 points
+    assert res["topo_shape"] == [2, 2, 2] and g["Np"] == [23, 23, 23] and g["L"] == [41, 41, 41]
+    out = d / "SAMPLES"
+    assert sorted(os.listdir(out)) == ["INFO-Sample_Test_1.h5", "Sample_Test_1.h5", "Sample_Test_1.xmf"]
+    (s,) = R.read_input_ScaRL("./input_ScaRL.txt", str(d)).samples
+    direct = _direct_blend(g, s)
+    with W.H5File(str(out / "Sample_Test_1.h5")) as f:
+        assert f.dataset_info("samples") == ((41, 41, 41), 64)
+        field = f.read()
+        assert np.array_equal(field, direct.astype(np.float64))
+        assert np.all(field > 0) and np.all(np.isfinite(field))           # lognormal marginal
+        assert list(f.read_attr("samples", "xMin", "d")) == [0.0, 0.0, 0.0]
+        assert list(f.read_attr("samples", "xMax", "d")) == [10.0, 10.0, 10.0]
+        assert list(f.read_attr("samples", "pos_N", "i")) == [41, 41, 41]
+        root = {n: f.read_attr("", n, "i").tolist() for n in ("nb_procs", "seedStart", "nFields", "Np", "Np_ovlp",
+                                                               "L", "method", "corrMod", "margiFirst")}
+        assert root == {"nb_procs": [8], "seedStart": [1], "nFields": [2, 2, 2], "Np": [23, 23, 23],
+                        "Np_ovlp": [5, 5, 5], "L": [41, 41, 41], "method": [1], "corrMod": [1], "margiFirst": [2]}
+        assert f.read_attr("", "xMaxGlob", "d").tolist() == [10.0, 10.0, 10.0]
+        assert f.read_attr("", "overlap", "d").tolist() == [1.0, 1.0, 1.0]
+        labels, times = f.read_times()
+        assert [l.rstrip() for l in labels] == LABELS and times.shape == (9, 8)
+        assert np.all(times >= 0)
+    xmf = open(out / "Sample_Test_1.xmf").read()
+    assert 'Dimensions="41 41 41"' in xmf and "./Sample_Test_1.h5:/samples" in xmf
+    assert xmf.count("              0.2500000000\n") == 3
+    with W.H5File(str(out / "INFO-Sample_Test_1.h5")) as f:
+        assert f.read_attr("", "Np", "i").tolist() == [23, 23, 23]
+    # a second run must not overwrite: the date-suffixed name of main_ScaRL.f90:425-446
+    when = datetime.datetime(2017, 1, 2, 3, 4, 5)
+    (res2,) = main_ScaRL.main(str(d), nb_procs=8, now=when)
+    assert res2["output_name"] == "Sample_Test_1-170102_030405"
+    assert (out / "Sample_Test_1-170102_030405.h5").exists()
+
+
+def test_general_bash_deck_file_per_rank(hip, tmp_path):
+    """extra_config 2x2x2, one file per rank, global .xmf; d_sample = 1 deletes the sample files."""
+    from scarl_amd import main_ScaRL, read_input as R, write_output as W
+    print("case: GENERAL_BASH deck, 2x2x2, file per rank", flush=True)
+    d = _deck("GENERAL_BASH", tmp_path, {"#SIZE#": "10"},
+              {"1 #Delete sample in the end": "0 #Delete sample in the end"})
+    (res,) = main_ScaRL.main(str(d), nb_procs=8)
+    g, out = res["geometry"], d / "SAMPLES"
+    names = sorted(os.listdir(out))
+    assert names == sorted(["INFO-Sample.h5", "Sample.xmf"] + [f"Sample_proc_{r:05d}.h5" for r in range(8)] +
+                           [f"Sample_proc_{r:05d}.xmf" for r in range(8)])
+    field = res["field"]
+    for r, blk in enumerate(res["blocks"]):
+        with W.H5File(str(out / f"Sample_proc_{r:05d}.h5")) as f:
+            got = f.read(f"sample_1_p{r}")
+            p0 = [v - 1 for v in blk.pos_0]
+            Np = g["Np"]
+            assert np.array_equal(got, field[p0[0]:p0[0] + Np[0], p0[1]:p0[1] + Np[1],
+                                             p0[2]:p0[2] + Np[2]].astype(np.float64))
+            assert f.read_attr(f"sample_1_p{r}", "pos_0", "i").tolist() == list(blk.pos_0)
+            assert f.read_attr("", "nFields", "i").tolist() == [2, 2, 2]
+            labels, times = f.read_times()
+            assert times.shape == (9, 1)
+    glob = open(out / "Sample.xmf").read()
+    assert glob.count("<Grid Name=\"Proc_") == 8 and "./Sample_proc_00007.h5:/sample_1_p7" in glob
+    # with the deck's own d_sample = 1 only the INFO file stays
+    d2 = _deck("GENERAL_BASH", tmp_path / "again", {"#SIZE#": "10"})
+    main_ScaRL.main(str(d2), nb_procs=8)
+    assert sorted(os.listdir(d2 / "SAMPLES")) == ["INFO-Sample.h5"]
+
+
+def test_fusion_deck_fft_mpi_global_field(hip, tmp_path):
+    """gen_meth 2 (FFT_MPI): no overlap, z slabs; L3 = 121 grows to 4 x 31 = 124."""
+    from scarl_amd import main_ScaRL, write_output as W
+    print("case: testFusion deck, FFT_MPI, 4 ranks", flush=True)
+    d = _deck("testFusion", tmp_path)
+    (res,) = main_ScaRL.main(str(d), nb_procs=4)
+    g = res["geometry"]
+    assert res["topo_shape"] == [1, 1, 4] and g["Np"] == [121, 121, 31] and g["L"] == [121, 121, 124]
+    with W.H5File(str(d / "SAMPLES" / "BASIC_Sample.h5")) as f:
+        assert f.dataset_info("samples") == ((121, 121, 124), 64)
+        x = f.read()
+        assert f.read_attr("", "Np_ovlp", "i").tolist() == [0, 0, 0]
+    # Gaussian marginal, mean 1, sd 1: the sample moments of this one field sit near them
+    assert abs(x.mean() - 1.0) < 0.25 and abs(x.std() - 1.0) < 0.25
+    assert np.array_equal(x, res["field"].astype(np.float64))

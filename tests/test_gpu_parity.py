@@ -347,6 +347,21 @@ def test_blend_subdomains(hip, prec, tol):
     assert mom[0, 2] == g.size
 
 
+def test_blend_wide_overlaps(hip):
+    """Overlaps wider than half a subdomain, as the reference's expansion rule allows them: with
+    two subdomains on an axis only Np >= Np_ovlp is required (main_ScaRL.f90:161); with three, the
+    middle one may be all ramp (Np = 2 Np_ovlp, :162)."""
+    from scarl_amd import generator
+    starts = [[0, 10], [0, 12, 24], [0]]
+    exts = [[40, 40], [24, 24, 24], [20]]
+    gdim = (50, 48, 20)
+    spec = generator.FieldSpec(n=(2, 2, 2), model=1, corrL=(6.0, 6.0, 6.0), seed=5, margi=1, avg=0.5, std_dev=2.0)
+    with generator.Batch.blend(gdim, starts, exts, spec, 64) as b:
+        b.run()
+        G = b.output(-1)
+    assert rel_l2(G, _blend_reference(gdim, starts, exts, spec)) <= FP64_TOL
+
+
 def test_blend_partition_of_unity_variance(hip):
     """With the same field on every subdomain the blend must give back sqrt(sum w) * f = f where
     the weights sum to one: check on a Gaussian field via sum of squared weights == 1."""
