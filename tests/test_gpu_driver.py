@@ -19,7 +19,9 @@ LABELS = ["Read_input", "Topology", "Generation", "Apply_unit", "Add_overlap", "
 
 def _deck(name, tmp_path, subst=None, config_subst=None):
     d = tmp_path / name
-    shutil.copytree(os.path.join(DECKS, name), d)
+    d.mkdir(parents=True)
+    for f in os.listdir(os.path.join(DECKS, name)):   # plain copies: writable whatever the source's mode
+        shutil.copyfile(os.path.join(DECKS, name, f), d / f)
     for fname, sub in (("input_ScaRL.txt", subst), ("config_ScaRL.txt", config_subst)):
         if sub:
             p = d / fname

@@ -16,7 +16,9 @@ DECKS = os.path.join(ROOT, "tests", "golden", "decks")
 
 def _deck(name, tmp_path, subst=None):
     d = tmp_path / name
-    shutil.copytree(os.path.join(DECKS, name), d)
+    d.mkdir(parents=True)
+    for f in os.listdir(os.path.join(DECKS, name)):   # plain copies: writable whatever the source's mode
+        shutil.copyfile(os.path.join(DECKS, name, f), d / f)
     if subst:
         p = d / "input_ScaRL.txt"
         txt = p.read_text()
