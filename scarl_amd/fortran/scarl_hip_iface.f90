@@ -3,10 +3,29 @@ module scarl_hip_iface
     use iso_c_binding
     implicit none
 
+    ! scarl_field_desc (include/scarl_hip.h); the member order gives the same C layout
+    type, bind(C) :: scarl_field_desc
+        integer(c_int32_t) :: n(3)
+        real(c_double) :: dx(3)
+        integer(c_int32_t) :: model
+        real(c_double) :: corrL(3)
+        real(c_double) :: nu
+        real(c_double) :: sigma
+        integer(c_int32_t) :: margi
+        real(c_double) :: avg, std_dev
+        integer(c_int64_t) :: seed
+        integer(c_int32_t) :: stream
+        integer(c_int32_t) :: normalize
+    end type
+    integer(c_int32_t), parameter :: SCARL_PREC_FP32 = 32, SCARL_PREC_FP64 = 64
+
     interface
         integer(c_int) function scarl_hip_init(device) bind(C, name="scarl_hip_init")
             import :: c_int, c_int32_t
             integer(c_int32_t), value :: device
+        end function
+        integer(c_int) function scarl_hip_finalize() bind(C, name="scarl_hip_finalize")
+            import :: c_int
         end function
         integer(c_int) function scarl_hip_last_error(buf, length) bind(C, name="scarl_hip_last_error")
             import :: c_int, c_int32_t, c_char
@@ -45,6 +64,43 @@ module scarl_hip_iface
             integer(c_int32_t), value :: n
             integer(c_int32_t), intent(inout) :: seed
             type(c_ptr), value :: x
+        end function
+        ! batched Hermitian generator (include/scarl_hip.h)
+        integer(c_int) function scarl_hip_batch_create(fields, count, precision, h) bind(C, name="scarl_hip_batch_create")
+            import :: c_int, c_int32_t, c_ptr, scarl_field_desc
+            type(scarl_field_desc), dimension(*), intent(in) :: fields
+            integer(c_int32_t), value :: count, precision
+            type(c_ptr), intent(out) :: h
+        end function
+        integer(c_int) function scarl_hip_blend_create(gdim, nb, starts, exts, params, precision, h) &
+                bind(C, name="scarl_hip_blend_create")
+            import :: c_int, c_int32_t, c_ptr, scarl_field_desc
+            integer(c_int32_t), dimension(3), intent(in) :: gdim, nb
+            integer(c_int32_t), dimension(*), intent(in) :: starts, exts
+            type(scarl_field_desc), intent(in) :: params
+            integer(c_int32_t), value :: precision
+            type(c_ptr), intent(out) :: h
+        end function
+        integer(c_int) function scarl_hip_batch_run(h, use_graph) bind(C, name="scarl_hip_batch_run")
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr), value :: h
+            integer(c_int32_t), value :: use_graph
+        end function
+        integer(c_int) function scarl_hip_batch_sync(h) bind(C, name="scarl_hip_batch_sync")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: h
+        end function
+        integer(c_int) function scarl_hip_batch_copy_output(h, field, dst, nbytes) &
+                bind(C, name="scarl_hip_batch_copy_output")
+            import :: c_int, c_int32_t, c_int64_t, c_ptr
+            type(c_ptr), value :: h
+            integer(c_int32_t), value :: field
+            type(c_ptr), value :: dst
+            integer(c_int64_t), value :: nbytes
+        end function
+        integer(c_int) function scarl_hip_batch_destroy(h) bind(C, name="scarl_hip_batch_destroy")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: h
         end function
     end interface
 
