@@ -55,7 +55,7 @@ def lib():
         L.oracle_r8vec_normal_01.argtypes = [C.c_int32, i32p, vp]
         L.oracle_r8vec_normal_01.restype = C.c_int
         L.oracle_philox4x32_10.argtypes = [vp, vp, vp]
-        L.oracle_gauss_at.argtypes = [C.c_long] * 5 + [C.c_uint64, C.c_uint32, vp]
+        L.oracle_gauss_at.argtypes = [C.c_long] * 6 + [C.c_uint64, C.c_uint32, vp]
         L.oracle_uniform53.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32]
         L.oracle_uniform53.restype = C.c_double
         L.oracle_fft1d.argtypes = [C.c_int, vp, C.c_int]
@@ -167,9 +167,9 @@ def philox4x32_10(ctr, key):
     return o
 
 
-def gauss_at(kx, ky, kz, h1, n2, seed, stream=0):
+def gauss_at(kx, ky, kz, h1, n2, n3, seed, stream=0):
     g = np.empty(2, np.float64)
-    lib().oracle_gauss_at(kx, ky, kz, h1, n2, seed, stream, _p(g))
+    lib().oracle_gauss_at(kx, ky, kz, h1, n2, n3, seed, stream, _p(g))
     return g
 
 

@@ -118,12 +118,15 @@ static void box_muller(uint32_t a, uint32_t b, double g[2]) {
     g[1] = rad * sin(2.0 * PI_EXACT * u2);
 }
 
-/* Standard-normal pair of half-spectrum point (kx, ky, kz): Philox counter = pair index
- * kx + h1 (ky + n2 floor(kz/2)); outputs (0,1) feed even kz, outputs (2,3) odd kz. */
-void oracle_gauss_at(long kx, long ky, long kz, long h1, long n2, uint64_t seed, uint32_t stream, double g[2]) {
+/* Standard-normal pair of half-spectrum point (kx, ky, kz). With h3 = ceil(n3/2), the points
+ * kz and kz + h3 of one z pencil share the Philox block at counter kx + h1 (ky + n2 (kz mod h3)):
+ * outputs (0,1) feed kz < h3, outputs (2,3) feed kz >= h3. */
+void oracle_gauss_at(long kx, long ky, long kz, long h1, long n2, long n3, uint64_t seed, uint32_t stream,
+                     double g[2]) {
     uint32_t o[4];
-    philox_at((uint64_t)(kx + h1 * (ky + n2 * (kz >> 1))), seed, stream, o);
-    if (kz & 1) box_muller(o[2], o[3], g);
+    const long h3 = (n3 + 1) / 2, hi = kz >= h3;
+    philox_at((uint64_t)(kx + h1 * (ky + n2 * (kz - hi * h3))), seed, stream, o);
+    if (hi) box_muller(o[2], o[3], g);
     else box_muller(o[0], o[1], g);
 }
 
@@ -633,7 +636,7 @@ void oracle_noise_hermitian(double *W, const int n[3], uint64_t seed, uint32_t s
                 double g[2];
                 int selfconj_plane = (kx == 0) || ((n1 % 2 == 0) && kx == n1 / 2);
                 if (!selfconj_plane) {
-                    oracle_gauss_at(kx, ky, kz, h1, n2, seed, stream, g);
+                    oracle_gauss_at(kx, ky, kz, h1, n2, n3, seed, stream, g);
                     W[2 * idx] = g[0] * rs2;
                     W[2 * idx + 1] = g[1] * rs2;
                     continue;
@@ -641,7 +644,7 @@ void oracle_noise_hermitian(double *W, const int n[3], uint64_t seed, uint32_t s
                 long py = (n2 - ky) % n2, pz = (n3 - kz) % n3;
                 long lin_self = ky + n2 * kz, lin_part = py + n2 * pz;
                 long cy = lin_self <= lin_part ? ky : py, cz = lin_self <= lin_part ? kz : pz;
-                oracle_gauss_at(kx, cy, cz, h1, n2, seed, stream, g);
+                oracle_gauss_at(kx, cy, cz, h1, n2, n3, seed, stream, g);
                 if (lin_self == lin_part) {
                     W[2 * idx] = g[0];
                     W[2 * idx + 1] = 0.0;

@@ -488,9 +488,8 @@ __device__ __forceinline__ cplx<T> box_muller(uint32_t a, uint32_t b) {
     }
 }
 
-// Standard-normal pair of spectral point with kz parity `half`, from the Philox call whose
-// counter is the point's pair index ctr = kx + h1 (ky + n2 floor(kz/2)): outputs (0,1) feed the
-// even kz, outputs (2,3) the odd kz, so one call serves two neighbouring points.
+// Standard-normal pair of a spectral point from the Philox block at counter ctr: outputs (0,1)
+// for half = 0, outputs (2,3) for half = 1, so one call serves two points.
 template <typename T>
 __device__ __forceinline__ cplx<T> gauss_at(uint64_t ctr, int half, uint64_t seed, uint32_t stream) {
     uint32_t o[4];

@@ -214,9 +214,66 @@ __device__ __forceinline__ void pencil_fft_fixed(const PlanDev& pl, const cplx<T
     }
 }
 
+// A pair generator for the first stage: gp(ctx, q, has_hi, lo, hi) produces the inputs q and
+// q + N/2 of a pencil.
+template <class F>
+struct GenPair {
+    F& f;
+    template <class C, typename T>
+    __device__ __forceinline__ void operator()(const C& c, int q, bool has_hi, cplx<T>& lo, cplx<T>& hi) const {
+        f(c, q, has_hi, lo, hi);
+    }
+};
+
+// Pencil-fastest two-stage FFT whose inputs are generated, not loaded: an even-radix first stage
+// holds the inputs q and q + N/2 in one butterfly (elements r and r + R0/2), so the generator
+// produces both from one call, straight into the butterfly's registers. The first stage's
+// outputs go through LDS to the second stage, which stores to global memory.
+template <typename T, int N, int R0, int R1, int TPP, class Prep, class GStore, class GP>
+__device__ __forceinline__ void pencil_fft_fixed_gen(const PlanDev& pl, const cplx<T>* __restrict__ pool,
+                                                     cplx<T>* lds, int npen, Prep&& prep, GStore&& gst, GP&& gp) {
+    static_assert(R0 * R1 == N && R0 % 2 == 0, "the first stage must have an even radix");
+    constexpr int TT = kBlock / TPP;
+    constexpr int nb = N / R0;
+    constexpr int BMAX = (nb + TPP - 1) / TPP;
+    const int tid = threadIdx.x;
+    const int p = tid % TT, b = tid / TT;
+    const bool active = p < npen;
+    const auto ctx = prep(active ? p : 0);
+    auto ld_l = [&](int n) -> cplx<T> { return lds[n * TT + p]; };
+    auto st_l = [&](int n, cplx<T> v) { lds[n * TT + p] = v; };
+    auto st_g = [&](int n, cplx<T> v) { gst(ctx, n, v); };
+    cplx<T> v[BMAX][R0];
+#pragma unroll
+    for (int i = 0; i < BMAX; ++i) {
+        const int j = b + TPP * i;
+        if (active && (nb % TPP == 0 || j < nb)) {
+#pragma unroll
+            for (int r = 0; r < R0 / 2; ++r) gp(ctx, j + r * nb, true, v[i][r], v[i][r + R0 / 2]);
+            dft<R0, T>(v[i]);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < BMAX; ++i) {
+        const int j = b + TPP * i;
+        if (active && (nb % TPP == 0 || j < nb)) {
+#pragma unroll
+            for (int r = 0; r < R0; ++r) {
+                if constexpr (R1 == 1) st_g(j * R0 + r, v[i][r]);
+                else st_l(j * R0 + r, v[i][r]);
+            }
+        }
+    }
+    if constexpr (R1 > 1) {
+        __syncthreads();
+        stage_fixed<R1, N, R0, TPP>(pool + pl.tw_off[1], b, active, false, ld_l, st_g);
+    }
+}
+
 // FFT policies: the pass kernels take one of these as a template parameter.
 template <int EC>
 struct GenericFft {
+    static constexpr bool kDirectGen = false;
     template <typename T, int LAYOUT, bool SIN, bool SOUT, class... A>
     __device__ __forceinline__ static void run(const PlanDev& pl, const cplx<T>* __restrict__ pool, cplx<T>* lds,
                                                int npen, A&&... a) {
@@ -225,6 +282,14 @@ struct GenericFft {
 };
 template <int N, int R0, int R1, int TPP>
 struct FixedFft {
+    static constexpr bool kDirectGen = R0 % 2 == 0;
+    template <typename T, class Prep, class GStore, class GP>
+    __device__ __forceinline__ static void run_gen(const PlanDev& pl, const cplx<T>* __restrict__ pool, cplx<T>* lds,
+                                                   int npen, Prep&& prep, GStore&& gst, GP&& gp) {
+        if constexpr (kDirectGen)
+            pencil_fft_fixed_gen<T, N, R0, R1, TPP>(pl, pool, lds, npen, static_cast<Prep&&>(prep),
+                                                    static_cast<GStore&&>(gst), static_cast<GP&&>(gp));
+    }
     template <typename T, int LAYOUT, bool SIN, bool SOUT, class... A>
     __device__ __forceinline__ static void run(const PlanDev& pl, const cplx<T>* __restrict__ pool, cplx<T>* lds,
                                                int npen, A&&... a) {

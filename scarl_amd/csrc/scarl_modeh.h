@@ -31,33 +31,38 @@ __device__ __forceinline__ T amplitude_h(const FieldH& f, int kx, int ky, int kz
 // Hermitian white noise at half-spectrum point (kx, ky, kz).  Off the self-conjugate planes
 // (kx = 0, kx = n1/2) W = (g1 + i g2)/sqrt2 of the point itself; on them W(kx,-ky,-kz) =
 // conj W(kx,ky,kz), drawn at the member of the pair with the lower linear index, and the
-// self-conjugate points are real g1.  The draw of point (kx, ky, kz) is gauss_at with
-// counter kx + h1 (ky + n2 floor(kz/2)) and half = kz mod 2.
+// self-conjugate points are real g1.  With h3 = ceil(n3/2), the draw of point (kx, ky, kz) is
+// gauss_at with counter kx + h1 (ky + n2 (kz mod h3)) and half = (kz >= h3): the points kz and
+// kz + h3 of a z pencil share one Philox block, and they are two inputs of the same butterfly of
+// an even-radix first stage along z.
 template <typename T>
 __device__ __forceinline__ cplx<T> noise_h(const FieldH& f, int kx, int ky, int kz) {
     const T rs2 = T(0.70710678118654752440084436210484904);
     const int n1 = f.n[0], n2 = f.n[1], n3 = f.n[2];
     const int64_t h1 = f.h1;
+    const int h3 = (n3 + 1) >> 1;
     const bool selfconj = (kx == 0) || (((n1 & 1) == 0) && kx == n1 / 2);
     if (!selfconj) {
-        const cplx<T> g = gauss_at<T>(kx + h1 * (ky + (int64_t)n2 * (kz >> 1)), kz & 1, f.seed, f.stream);
+        const int hz = kz >= h3;
+        const cplx<T> g = gauss_at<T>(kx + h1 * (ky + (int64_t)n2 * (kz - hz * h3)), hz, f.seed, f.stream);
         return {g.x * rs2, g.y * rs2};
     }
     const int py = ky ? n2 - ky : 0, pz = kz ? n3 - kz : 0;
     const int64_t ls = ky + (int64_t)n2 * kz, lp = py + (int64_t)n2 * pz;
     const int cy = ls <= lp ? ky : py, cz = ls <= lp ? kz : pz;
-    const cplx<T> g = gauss_at<T>(kx + h1 * (cy + (int64_t)n2 * (cz >> 1)), cz & 1, f.seed, f.stream);
+    const int hc = cz >= h3;
+    const cplx<T> g = gauss_at<T>(kx + h1 * (cy + (int64_t)n2 * (cz - hc * h3)), hc, f.seed, f.stream);
     if (ls == lp) return {g.x, T(0)};
     return {g.x * rs2, (ls < lp ? g.y : -g.y) * rs2};
 }
 
-// Two neighbouring points (kx, ky, kz0) and (kx, ky, kz0 + 1), kz0 even, off the self-conjugate
-// planes: one Philox call.
+// The two points (kx, ky, q) and (kx, ky, q + h3), q < h3, off the self-conjugate planes: one
+// Philox call.
 template <typename T>
-__device__ __forceinline__ void noise_h_pair(const FieldH& f, int kx, int ky, int kz0, cplx<T>& w0, cplx<T>& w1) {
+__device__ __forceinline__ void noise_h_pair(const FieldH& f, int kx, int ky, int q, cplx<T>& w0, cplx<T>& w1) {
     const T rs2 = T(0.70710678118654752440084436210484904);
     uint32_t o[4];
-    philox_at(kx + (int64_t)f.h1 * (ky + (int64_t)f.n[1] * (kz0 >> 1)), f.seed, f.stream, o);
+    philox_at(kx + (int64_t)f.h1 * (ky + (int64_t)f.n[1] * q), f.seed, f.stream, o);
     const cplx<T> g0 = box_muller<T>(o[0], o[1]), g1 = box_muller<T>(o[2], o[3]);
     w0 = {g0.x * rs2, g0.y * rs2};
     w1 = {g1.x * rs2, g1.y * rs2};
@@ -111,15 +116,16 @@ __global__ __launch_bounds__(kBlock) void k_h_strided(const FieldH* __restrict__
         return gload(ws + (c.base + n * S));
     };
     auto gst = [&](const Ctx& c, int n, cplx<T> v) { gstore(ws + (c.base + n * S), v); };
-    // stage-in of the generating pass: along z, neighbouring kz share one Philox call
-    auto gin = [&](const Ctx& c, int b, int tpp, auto&& put) {
-        const int N = pl.N;
-        const bool paired = gen && axis == 2 && f.src_mode == SRC_DEVICE_NOISE && c.kx != 0 &&
-                            !(((f.n[0] & 1) == 0) && c.kx == f.n[0] / 2);
-        if (paired) {
+    // Generation along z: the points q and q + h3 (h3 = ceil(n3/2)) of a pencil share one Philox
+    // call (noise_h). gpair_dev produces both for device noise off the self-conjugate planes;
+    // gpair falls back to one gld per point on those planes, for host-exported noise and for a
+    // workspace source.
+    auto gpair_dev = [&](const Ctx& c, int q, bool has_hi, cplx<T>& lo, cplx<T>& hi) {
+        const int N = pl.N, h3 = (N + 1) >> 1;
+        {
             if constexpr (sizeof(T) == 4) {
                 // amp(k) W(k) = sqrt(-ln2 amp0^2 log2(u1) P(q)) (cos 2 pi u2, sin 2 pi u2) with
-                // P = (1 + q)^(2 expo) (Matern family) or 2^(-q / (4 ln 2)) (Gaussian), q = sum_d
+                // P = (1 + s)^(2 expo) (Matern family) or 2^(-s / (4 ln 2)) (Gaussian), s = sum_d
                 // (alpha_d f_d)^2: amplitude, the 1/sqrt2 of the complex draw and the Box-Muller
                 // radius share one square root; |f_z| = min(kz, n3 - kz).
                 const float ax = float(f.alpha[0]) * float(c.kx);
@@ -129,46 +135,78 @@ __global__ __launch_bounds__(kBlock) void k_h_strided(const FieldH* __restrict__
                 const float az = float(f.alpha[2]);
                 const float e2 = gauss ? -0.36067376022224085f : float(2.0 * f.expo);
                 const float cc = float(-0.69314718055994530942 * f.amp0 * f.amp0);
-                const int n3 = f.n[2];
-                const int64_t ctr0 = c.kx + (int64_t)f.h1 * c.k2, cstep = (int64_t)f.h1 * f.n[1];
                 auto root = [&](int kz, uint32_t u) {
-                    const float t = az * float(min(kz, n3 - kz));
-                    const float s = fmaf(t, t, qxy);
-                    const float P = __builtin_amdgcn_exp2f(e2 * (gauss ? s : __builtin_amdgcn_logf(s)));
+                    const float t = az * float(min(kz, N - kz));
+                    const float s2 = fmaf(t, t, qxy);
+                    const float P = __builtin_amdgcn_exp2f(e2 * (gauss ? s2 : __builtin_amdgcn_logf(s2)));
                     return __builtin_amdgcn_sqrtf(cc * __builtin_amdgcn_logf(u24<float>(u)) * P);
                 };
-#pragma unroll 1
-                for (int q = b; 2 * q < N; q += tpp) {
-                    const int kz0 = 2 * q;
-                    uint32_t o[4];
-                    philox_at(ctr0 + cstep * q, f.seed, f.stream, o);
-                    const float r0 = root(kz0, o[0]), v0 = u24<float>(o[1]);
-                    put(kz0, cplx<T>{r0 * __builtin_amdgcn_cosf(v0), r0 * __builtin_amdgcn_sinf(v0)});
-                    if (kz0 + 1 < N) {
-                        const float r1 = root(kz0 + 1, o[2]), v1 = u24<float>(o[3]);
-                        put(kz0 + 1, cplx<T>{r1 * __builtin_amdgcn_cosf(v1), r1 * __builtin_amdgcn_sinf(v1)});
-                    }
+                uint32_t o[4];
+                philox_at(c.kx + (int64_t)f.h1 * (c.k2 + (int64_t)f.n[1] * q), f.seed, f.stream, o);
+                const float r0 = root(q, o[0]), v0 = u24<float>(o[1]);
+                lo = cplx<T>{r0 * __builtin_amdgcn_cosf(v0), r0 * __builtin_amdgcn_sinf(v0)};
+                if (has_hi) {
+                    const float r1 = root(q + h3, o[2]), v1 = u24<float>(o[3]);
+                    hi = cplx<T>{r1 * __builtin_amdgcn_cosf(v1), r1 * __builtin_amdgcn_sinf(v1)};
                 }
             } else {
-#pragma unroll 1
-                for (int q = b; 2 * q < N; q += tpp) {
-                    const int kz0 = 2 * q;
-                    cplx<T> w0, w1;
-                    noise_h_pair<T>(f, c.kx, c.k2, kz0, w0, w1);
-                    const T a0 = amplitude_h<T>(f, c.kx, c.k2, kz0);
-                    put(kz0, cplx<T>{a0 * w0.x, a0 * w0.y});
-                    if (kz0 + 1 < N) {
-                        const T a1 = amplitude_h<T>(f, c.kx, c.k2, kz0 + 1);
-                        put(kz0 + 1, cplx<T>{a1 * w1.x, a1 * w1.y});
-                    }
+                cplx<T> w0, w1;
+                noise_h_pair<T>(f, c.kx, c.k2, q, w0, w1);
+                const T a0 = amplitude_h<T>(f, c.kx, c.k2, q);
+                lo = cplx<T>{a0 * w0.x, a0 * w0.y};
+                if (has_hi) {
+                    const T a1 = amplitude_h<T>(f, c.kx, c.k2, q + h3);
+                    hi = cplx<T>{a1 * w1.x, a1 * w1.y};
                 }
             }
-        } else {
-#pragma unroll 1
-            for (int n = b; n < N; n += tpp) put(n, gld(c, n));
         }
     };
-    Fft::template run<T, LAYOUT_PF, GEN, false>(pl, pool, lds, npen, prep, gld, gst, gin);
+    auto gpair = [&](const Ctx& c, int q, bool has_hi, cplx<T>& lo, cplx<T>& hi) {
+        const int h3 = (pl.N + 1) >> 1;
+        const bool paired = gen && axis == 2 && f.src_mode == SRC_DEVICE_NOISE && c.kx != 0 &&
+                            !(((f.n[0] & 1) == 0) && c.kx == f.n[0] / 2);
+        if (paired) {
+            gpair_dev(c, q, has_hi, lo, hi);
+        } else {
+            lo = gld(c, q);
+            if (has_hi) hi = gld(c, q + h3);
+        }
+    };
+    if constexpr (GEN) {
+        if (axis == 2) {
+            // Tiles whose pencils are all off the self-conjugate planes (no kx = 0 pencil, not the
+            // Nyquist plane) with device noise: the first stage's butterflies hold the pairs
+            // (q, q + N/2), generated straight into registers. The test is uniform over the tile,
+            // so only the paired draw is unrolled into the butterflies. Other tiles stage the
+            // generated pencils through LDS.
+            const int kx_first = tl.pencil0 % wp;
+            const bool direct = gen && !nyq && f.src_mode == SRC_DEVICE_NOISE && kx_first != 0 &&
+                                kx_first + npen <= wp;
+            if constexpr (Fft::kDirectGen) {
+                if (direct) {
+                    Fft::template run_gen<T>(pl, pool, lds, npen, prep, gst, GenPair<decltype(gpair_dev)>{gpair_dev});
+                    return;
+                }
+            }
+            {
+                // stage the generated pencil through LDS
+                auto gin = [&](const Ctx& c, int b, int tpp, auto&& put) {
+                    const int N = pl.N, h3 = (N + 1) >> 1;
+#pragma unroll 1
+                    for (int q = b; q < h3; q += tpp) {
+                        cplx<T> lo, hi;
+                        const bool has_hi = q + h3 < N;
+                        gpair(c, q, has_hi, lo, hi);
+                        put(q, lo);
+                        if (has_hi) put(q + h3, hi);
+                    }
+                };
+                Fft::template run<T, LAYOUT_PF, true, false>(pl, pool, lds, npen, prep, gld, gst, gin);
+            }
+            return;
+        }
+    }
+    Fft::template run<T, LAYOUT_PF, GEN, false>(pl, pool, lds, npen, prep, gld, gst);
 }
 
 // x pass: for even n1 a half-length complex FFT of

@@ -227,6 +227,21 @@ def test_hermitian_fp32_device_noise(hip):
     assert rel_l2(gpu, ref) <= FP32_TOL
 
 
+@pytest.mark.parametrize("prec,n", [(32, (16, 12, 64)), (32, (15, 8, 96)), (32, (8, 8, 128)), (32, (12, 10, 160)),
+                                    (32, (10, 8, 176)), (32, (8, 8, 256)), (32, (8, 6, 512)), (64, (9, 6, 256)),
+                                    (64, (8, 4, 512))])
+def test_hermitian_device_noise_generated_in_first_stage(hip, prec, n):
+    """z lengths with a compile-time plan whose first radix is even: the pairs (q, q + n3/2) are
+    generated straight into the first butterfly's registers. Both self-conjugate planes (kx = 0
+    and, for even n1, kx = n1/2) take the per-point path inside the same tiles."""
+    from scarl_amd import generator
+    _say(f"hermitian device noise fp{prec} n={n}")
+    L = (4.0, 3.0, 24.0)
+    spec = generator.FieldSpec(n=n, model=2, corrL=L, nu=0.3, seed=19, stream=4, margi=1, avg=0.5, std_dev=2.0)
+    ref = O.gen_mode_h(n, (1, 1, 1), 2, L, 0.3, seed=19, stream=4, margi=1, avg=0.5, std=2.0)
+    assert rel_l2(generator.gen_field_h(spec, prec), ref) <= (FP32_TOL if prec == 32 else FP64_TOL)
+
+
 @pytest.mark.parametrize("n", [(23, 12, 10), (41, 41, 41), (81, 30, 17), (5, 6, 1), (45, 33, 1)])
 def test_hermitian_odd_x_extent(hip, n):
     """Odd x extents (the reference geometry's L = 1 + ceiling(...) is usually odd): rows of
