@@ -128,3 +128,41 @@ def test_fusion_deck_fft_mpi_global_field(hip, tmp_path):
     # Gaussian marginal, mean 1, sd 1: the sample moments of this one field sit near them
     assert abs(x.mean() - 1.0) < 0.25 and abs(x.std() - 1.0) < 0.25
     assert np.array_equal(x, res["field"].astype(np.float64))
+
+
+def test_stat_program_on_a_written_sample(hip, tmp_path):
+    """main_Stat on a sample the deck program wrote: statistics on the sample file's root and
+    STAT_<name> with the copied attributes, against the oracle's corrL estimator on the data
+    read back."""
+    import io
+    from oracle import oracle as O
+    from scarl_amd import main_ScaRL, main_Stat, write_output as W
+    print("case: Stat program on the 3_TEST sample, 1 rank", flush=True)
+    d = _deck("3_TEST", tmp_path)
+    main_ScaRL.main(str(d), nb_procs=1)
+    path = str(d / "SAMPLES" / "Sample_Test_1.h5")
+    with W.H5File(path) as f:
+        x = f.read()
+    (STA,) = main_Stat.main(io.StringIO(f'1 #Number of Samples\n1 #corrL\n0 #delete\n"{path}"\n"samples"\n'))
+    avg = x.mean()
+    sd = np.sqrt((x * x).mean() - avg * avg)
+    ref_sk = O.rebuild_sk_corrl(x, 0.25, avg, sd)
+    ref_corrl = [O.corrl_from_sk(ref_sk[a][:x.shape[a] // 2], 0.25) for a in range(3)]
+    for p in (path, str(d / "SAMPLES" / "STAT_Sample_Test_1.h5")):
+        with W.H5File(p) as f:
+            assert abs(f.read_attr("", "globalAvg", "d")[0] - avg) <= 1e-12 * abs(avg)
+            assert abs(f.read_attr("", "globalStdDev", "d")[0] - sd) <= 1e-10 * sd
+            np.testing.assert_allclose(f.read_attr("", "corrL_out", "d"), ref_corrl, rtol=1e-10)
+            for a in range(3):
+                sk = f.read_attr("", f"Sk_{a + 1}", "d")
+                assert sk.size == x.shape[a] // 2
+                np.testing.assert_allclose(sk, ref_sk[a][:sk.size], rtol=0, atol=1e-11 * np.abs(ref_sk[a]).max())
+    with W.H5File(str(d / "SAMPLES" / "STAT_Sample_Test_1.h5")) as f:
+        assert f.read_attr("", "L", "i").tolist() == [41, 41, 41]
+        assert f.read_attr("", "seedStart", "i").tolist() == [1]
+        assert f.read_attr("", "old_file_mb_size", "d")[0] > 0
+    # deleteSampleInTheEnd = 1 removes the sample's files and keeps STAT_
+    (STA2,) = main_Stat.main(io.StringIO(f'1\n0\n1\n"{path}"\n"samples"\n'))
+    left = sorted(os.listdir(d / "SAMPLES"))
+    assert "Sample_Test_1.h5" not in left and "Sample_Test_1.xmf" not in left and "STAT_Sample_Test_1.h5" in left
+    assert STA2.corrL_out is None

@@ -8,6 +8,8 @@ bash_ScaRL/bash_Scarl.sh:80-82).
 import os
 import shutil
 
+import numpy as np
+
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -178,3 +180,33 @@ def test_find_factors_and_topology_choice():
     # FFT_MPI: FFTW slabs of ceiling(51 / 16) = 4 planes leave 13 ranks with planes
     assert G.decide_topo_shape(16, [51, 51, 51], [0, 0, 0], gen_meth=G.FFT_MPI) == [1, 1, 13]
     assert G.decide_topo_shape(8, [51, 51, 51], [0, 0, 0], gen_meth=G.FFT_MPI) == [1, 1, 8]
+
+
+def test_stat_inputs(tmp_path):
+    """The Stat program's stdin decks: testFusion's holds the dataset name; the pointsTest one
+    stops after the path, so the dataset read meets the end of the input, as the reference's
+    READ would."""
+    import io
+    from scarl_amd import main_Stat, read_input as R
+    lines = open(os.path.join(DECKS, "testFusion", "input_stat.txt")).read().splitlines()
+    assert [R.list_read(lines[0], "i")[0], R.list_read(lines[1], "i")[0], R.list_read(lines[2], "i")[0]] == [1, 1, 0]
+    assert R.list_read(lines[3], "a")[0] == "/workdir/carvalho/RF_Test/BASIC_TEST_ScaRL/SAMPLES/BASIC_Sample.h5"
+    assert R.list_read(lines[4], "a")[0] == "samples"
+    with pytest.raises(R.DeckError, match="end of input"):
+        main_Stat.main(open(os.path.join(DECKS, "5p_3TEST", "input_stat")))
+    with pytest.raises(R.DeckError, match="bad integer"):
+        main_Stat.main(io.StringIO("one\n"))
+
+
+def test_constant_field_sample_needs_no_gpu(tmp_path):
+    """std_dev = 0 (main_ScaRL.f90:145-155): a 2x2x2 constant field through
+    write_constant_field, before any generation."""
+    from scarl_amd import main_ScaRL, write_output as W
+    d = _deck("3_TEST", tmp_path)
+    p = d / "input_ScaRL.txt"
+    p.write_text(p.read_text().replace("1d0             #Standard Deviation", "0d0             #Standard Deviation"))
+    (res,) = main_ScaRL.main(str(d), nb_procs=4)
+    assert res == {"output_name": "Sample_Test_1", "constant": True}
+    with W.H5File(str(d / "SAMPLES" / "Sample_Test_1.h5")) as f:
+        assert np.array_equal(f.read(), np.full((2, 2, 2), 1.0))
+        assert f.read_attr("", "xStep", "d").tolist() == [10.0, 10.0, 10.0]
