@@ -134,7 +134,7 @@ int scarl_hip_target_corr(int32_t model, double nu, const double* h_rho, int32_t
 
 /* ---- Hermitian spectral generator (mode H), single field and batches ---------------------- */
 typedef struct scarl_field_desc {
-    int32_t n[3];          /* extents (x, y, z); n[2] = 1 for a 2D field; n[0] must be even      */
+    int32_t n[3];          /* extents (x, y, z); n[2] = 1 for a 2D field; any n[0], odd or even  */
     double dx[3];          /* grid spacing                                                       */
     int32_t model;         /* SCARL_CM_*                                                         */
     double corrL[3];       /* correlation lengths, same units as dx                             */
