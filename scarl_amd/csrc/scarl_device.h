@@ -466,6 +466,15 @@ __device__ __forceinline__ void philox_at(uint64_t idx, uint64_t seed, uint32_t 
     philox4x32_10(o, static_cast<uint32_t>(seed), static_cast<uint32_t>(seed >> 32));
 }
 
+// Within each 16-lane row of a wave, lane j receives the value of lane (16 - j) mod 16: DPP
+// row_mirror (lane j <- lane 15 - j), then DPP row_shr:1 (lane j <- lane j - 1). Lane 0 keeps the
+// mirrored value of lane 15 and must be overridden by the caller. Every lane of the row must
+// execute it.
+__device__ __forceinline__ float row_partner16(float v) {
+    const int m = __builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x140, 0xF, 0xF, false);
+    return __int_as_float(__builtin_amdgcn_update_dpp(m, m, 0x111, 0xF, 0xF, false));
+}
+
 template <typename T>
 __device__ __forceinline__ T u24(uint32_t x) {
     return static_cast<T>(x >> 8) * static_cast<T>(5.9604644775390625e-08) + static_cast<T>(2.98023223876953125e-08);

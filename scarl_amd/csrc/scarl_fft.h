@@ -270,10 +270,68 @@ __device__ __forceinline__ void pencil_fft_fixed_gen(const PlanDev& pl, const cp
     }
 }
 
+// Butterfly-fastest two-stage FFT for the x pass of an even-length real output (FP32, 16
+// butterflies of the first stage, one per thread of a 16-lane row). Input k of the half-length
+// transform combines X_k with X_{M-k} (M = N). For j >= 1 the partner of element r of butterfly j
+// is element R0 - 1 - r of butterfly 16 - j, so each row loads every X_k once (aligned) and gets
+// the partners from the mirrored lane (row_partner16). Lane 0's partners are its own elements
+// R0 - r, and the Nyquist value for r = 0.
+// xld(ctx, k) loads X_k, xnyq(ctx) the Nyquist value, xcomb(k, X_k, X_{M-k}) forms the input.
+template <typename T, bool SOUT, int N, int R0, int R1, int TPP, class Prep, class XLoad, class XNyq, class XComb,
+          class GStore>
+__device__ __forceinline__ void pencil_fft_fixed_xpair(const PlanDev& pl, const cplx<T>* __restrict__ pool,
+                                                       cplx<T>* lds, int npen, Prep&& prep, XLoad&& xld,
+                                                       XNyq&& xnyq, XComb&& xcomb, GStore&& gst) {
+    static_assert(sizeof(T) == 4 && TPP == 16 && N / R0 == 16 && R1 > 1, "16 first-stage butterflies per row");
+    constexpr int nb = 16;
+    constexpr int LDP = N + N / 16 + 1;
+    const int tid = threadIdx.x, b = tid % TPP, p = tid / TPP;
+    const bool active = p < npen;
+    auto addr = [&](int n) -> int { return p * LDP + n + (n >> 4); };
+    const auto ctx = prep(active ? p : 0);
+    auto ld_l = [&](int n) -> cplx<T> { return lds[addr(n)]; };
+    auto st_l = [&](int n, cplx<T> v) { lds[addr(n)] = v; };
+    auto st_g = [&](int n, cplx<T> v) { gst(ctx, n, v); };
+    cplx<T> a[R0], v[R0];
+    cplx<T> xn = {T(0), T(0)};
+    if (active) {
+#pragma unroll
+        for (int r = 0; r < R0; ++r) a[r] = xld(ctx, b + nb * r);
+        if (b == 0) xn = xnyq(ctx);
+    } else {
+#pragma unroll
+        for (int r = 0; r < R0; ++r) a[r] = cplx<T>{T(0), T(0)};
+    }
+#pragma unroll
+    for (int r = 0; r < R0; ++r) {
+        const cplx<T> m = a[R0 - 1 - r];
+        cplx<T> q = {row_partner16(m.x), row_partner16(m.y)};
+        if (b == 0) q = r ? a[(R0 - r) % R0] : xn;
+        v[r] = xcomb(b + nb * r, a[r], q);
+    }
+    dft<R0, T>(v);
+    if (active) {
+#pragma unroll
+        for (int r = 0; r < R0; ++r) st_l(b * R0 + r, v[r]);
+    }
+    __syncthreads();
+    const cplx<T>* tw = pool + pl.tw_off[1];
+    if constexpr (SOUT) {
+        stage_fixed<R1, N, R0, TPP>(tw, b, active, true, ld_l, st_l);
+        __syncthreads();
+#pragma unroll 1
+        for (int n = b; n < N; n += TPP)
+            if (active) st_g(n, ld_l(n));
+    } else {
+        stage_fixed<R1, N, R0, TPP>(tw, b, active, false, ld_l, st_g);
+    }
+}
+
 // FFT policies: the pass kernels take one of these as a template parameter.
 template <int EC>
 struct GenericFft {
     static constexpr bool kDirectGen = false;
+    static constexpr bool kXPair = false;
     template <typename T, int LAYOUT, bool SIN, bool SOUT, class... A>
     __device__ __forceinline__ static void run(const PlanDev& pl, const cplx<T>* __restrict__ pool, cplx<T>* lds,
                                                int npen, A&&... a) {
@@ -283,6 +341,14 @@ struct GenericFft {
 template <int N, int R0, int R1, int TPP>
 struct FixedFft {
     static constexpr bool kDirectGen = R0 % 2 == 0;
+    static constexpr bool kXPair = TPP == 16 && N / R0 == 16 && R1 > 1;
+    template <typename T, bool SOUT, class Prep, class XLoad, class XNyq, class XComb, class GStore>
+    __device__ __forceinline__ static void run_xpair(const PlanDev& pl, const cplx<T>* __restrict__ pool,
+                                                     cplx<T>* lds, int npen, Prep&& prep, XLoad&& xld, XNyq&& xnyq,
+                                                     XComb&& xcomb, GStore&& gst) {
+        if constexpr (kXPair && sizeof(T) == 4)
+            pencil_fft_fixed_xpair<T, SOUT, N, R0, R1, TPP>(pl, pool, lds, npen, prep, xld, xnyq, xcomb, gst);
+    }
     template <typename T, class Prep, class GStore, class GP>
     __device__ __forceinline__ static void run_gen(const PlanDev& pl, const cplx<T>* __restrict__ pool, cplx<T>* lds,
                                                    int npen, Prep&& prep, GStore&& gst, GP&& gp) {

@@ -67,6 +67,7 @@ __device__ __forceinline__ float sin16(int m) { return cos16(m - 4); }
 
 struct MfmaFft256 {
     static constexpr bool kDirectGen = false;
+    static constexpr bool kXPair = false;
     template <typename T, int LAYOUT, bool SIN, bool SOUT, class Prep, class GLoad, class GStore,
               class GIn = DefaultStageIn>
     __device__ __forceinline__ static void run(const PlanDev& pl, const cplx<T>* __restrict__ pool, cplx<T>* lds,

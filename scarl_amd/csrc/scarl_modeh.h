@@ -327,7 +327,24 @@ __global__ __launch_bounds__(kBlock) void k_h_xpass(const FieldH* __restrict__ f
     };
     // the blend epilogue (PoU weights, owner order) is staged through LDS; the plain one stores from
     // registers in the last stage
-    Fft::template run<T, LAYOUT_BF, false, BLEND>(pl, pool, lds, npen, prep, gld, gst);
+    if constexpr (!FULL && Fft::kXPair && sizeof(T) == 4) {
+        // every X_k loaded once; X_{M-k} comes from the mirrored lane of the row
+        auto xld = [&](const Ctx& c, int k) -> cplx<T> { return gload(c.X + k); };
+        auto xnyq = [&](const Ctx& c) -> cplx<T> { return gload(c.Xn); };
+        auto xcomb = [&](int k, cplx<T> a, cplx<T> bb) -> cplx<T> {
+            if (k == 0) {
+                a.y = T(0);
+                bb.y = T(0);
+            }
+            bb.y = -bb.y;
+            const cplx<T> s = a + bb, d = a - bb;
+            const cplx<T> wd = cmul(pre[k], d);
+            return {s.x - wd.y, s.y + wd.x};
+        };
+        Fft::template run_xpair<T, BLEND>(pl, pool, lds, npen, prep, xld, xnyq, xcomb, gst);
+    } else {
+        Fft::template run<T, LAYOUT_BF, false, BLEND>(pl, pool, lds, npen, prep, gld, gst);
+    }
     if (mom) {
         __syncthreads();
         block_reduce_mom(ms, mc, ms2, mc2, reinterpret_cast<double*>(smem), partials + tix);

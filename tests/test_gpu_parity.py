@@ -229,7 +229,7 @@ def test_hermitian_fp32_device_noise(hip):
 
 @pytest.mark.parametrize("prec,n", [(32, (16, 12, 64)), (32, (15, 8, 96)), (32, (8, 8, 128)), (32, (12, 10, 160)),
                                     (32, (10, 8, 176)), (32, (8, 8, 256)), (32, (8, 6, 512)), (64, (9, 6, 256)),
-                                    (64, (8, 4, 512))])
+                                    (64, (8, 4, 512)), (32, (512, 6, 4)), (32, (512, 5, 1))])
 def test_hermitian_device_noise_generated_in_first_stage(hip, prec, n):
     """z lengths with a compile-time plan whose first radix is even: the pairs (q, q + n3/2) are
     generated straight into the first butterfly's registers. Both self-conjugate planes (kx = 0
@@ -375,6 +375,20 @@ def test_blend_wide_overlaps(hip):
         b.run()
         G = b.output(-1)
     assert rel_l2(G, _blend_reference(gdim, starts, exts, spec)) <= FP64_TOL
+
+
+def test_blend_x_extent_512(hip):
+    """Subdomains 512 wide along x: the FP32 x pass takes the row-mirrored pair loads (each X_k
+    loaded once, X_{M-k} from the mirrored lane) with the blend epilogue."""
+    from scarl_amd import generator
+    starts = [[0, 448], [0, 4], [0]]
+    exts = [[512, 512], [8, 8], [6]]
+    gdim = (960, 12, 6)
+    spec = generator.FieldSpec(n=(2, 2, 2), model=3, corrL=(40.0, 3.0, 3.0), seed=8, margi=2, avg=1.0, std_dev=0.5)
+    with generator.Batch.blend(gdim, starts, exts, spec, 32) as b:
+        b.run()
+        G = b.output(-1)
+    assert rel_l2(G, _blend_reference(gdim, starts, exts, spec)) <= FP32_TOL
 
 
 def test_blend_partition_of_unity_variance(hip):
