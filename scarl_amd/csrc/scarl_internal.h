@@ -119,6 +119,10 @@ struct Tile {
 // bf16 MFMA radix-16 stages (scarl_mfma.h), selected by SCARL_TUNE_MFMA_DFT16.
 constexpr int kSpecMfma256 = 64;
 
+// Blocks per field of the tile-partial reduction (k_mom_field); the per-field buffer holds
+// kMomChunks partials per field.
+constexpr int kMomChunks = 32;
+
 // Compensated (Neumaier) partial sums of x and x^2 for one tile.
 struct MomPartial {
     double s, c, s2, c2;

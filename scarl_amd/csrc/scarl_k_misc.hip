@@ -3,7 +3,8 @@
 
 namespace scarl {
 
-// Per-field reduction of tile partials -> field_mom[f]; then per slot -> slot_out[slot]
+// Per-field reduction of tile partials in kMomChunks blocks per field -> field_mom[f * kMomChunks + g];
+// then per slot -> slot_out[slot]
 // slot_out[slot] = {sum, sum2, count, mean, std} with the reference's std formula
 // sqrt(sum2/n - mean^2) (SRC_F/Stat_ScaRL/statistics_RF.f90:80-82).
 __global__ __launch_bounds__(kBlock) void k_mom_field(const FieldH* __restrict__ fields,
@@ -11,12 +12,12 @@ __global__ __launch_bounds__(kBlock) void k_mom_field(const FieldH* __restrict__
                                                        const MomPartial* __restrict__ partials,
                                                        MomPartial* __restrict__ field_mom) {
     __shared__ double red[4 * kBlock];
-    const int fi = blockIdx.x;
+    const int fi = blockIdx.x, g = blockIdx.y;
     const FieldH& f = fields[fi];
     double s = 0, c = 0, s2 = 0, c2 = 0;
     if (f.mom_slot >= 0) {
         const int2 r = field_tiles[fi];
-        for (int t = r.x + threadIdx.x; t < r.y; t += kBlock) {
+        for (int t = r.x + g * kBlock + threadIdx.x; t < r.y; t += kBlock * kMomChunks) {
             const MomPartial m = partials[t];
             nadd(s, c, m.s);
             c += m.c;
@@ -24,7 +25,7 @@ __global__ __launch_bounds__(kBlock) void k_mom_field(const FieldH* __restrict__
             c2 += m.c2;
         }
     }
-    block_reduce_mom(s, c, s2, c2, red, field_mom + fi);
+    block_reduce_mom(s, c, s2, c2, red, field_mom + fi * kMomChunks + g);
 }
 
 __global__ __launch_bounds__(kBlock) void k_mom_slot(const FieldH* __restrict__ fields, int nfields,
@@ -34,9 +35,9 @@ __global__ __launch_bounds__(kBlock) void k_mom_slot(const FieldH* __restrict__ 
     __shared__ double red[4 * kBlock];
     const int sl = blockIdx.x;
     double s = 0, c = 0, s2 = 0, c2 = 0;
-    for (int fi = threadIdx.x; fi < nfields; fi += kBlock) {
-        if (fields[fi].mom_slot != sl) continue;
-        const MomPartial m = field_mom[fi];
+    for (int i = threadIdx.x; i < nfields * kMomChunks; i += kBlock) {
+        if (fields[i / kMomChunks].mom_slot != sl) continue;
+        const MomPartial m = field_mom[i];
         nadd(s, c, m.s);
         c += m.c;
         nadd(s2, c2, m.s2);
@@ -304,7 +305,8 @@ hipError_t launch_normalize(double* x, int64_t n, double a, double b, hipStream_
 hipError_t launch_mom(const FieldH* fields, int nfields, const int2* field_tiles, const MomPartial* partials,
                       MomPartial* field_mom, const double* slot_count, int nslots, double* slot_out, hipStream_t s) {
     if (nfields <= 0 || nslots <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_mom_field, dim3(nfields), dim3(kBlock), 0, s, fields, field_tiles, partials, field_mom);
+    hipLaunchKernelGGL(k_mom_field, dim3(nfields, kMomChunks), dim3(kBlock), 0, s, fields, field_tiles, partials,
+                       field_mom);
     hipLaunchKernelGGL(k_mom_slot, dim3(nslots), dim3(kBlock), 0, s, fields, nfields, field_mom, slot_count, nslots,
                        slot_out);
     return hipGetLastError();

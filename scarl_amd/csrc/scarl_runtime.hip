@@ -840,7 +840,7 @@ static int finalize_batch(scarl_batch* b, const std::vector<double>& slot_count)
     SCARL_TRY(upload_vec(b->tx, (void**)&b->d_tx));
     SCARL_TRY(upload_vec(b->field_xtiles, (void**)&b->d_field_xtiles));
     SCARL_TRY(dmalloc((void**)&b->d_partials, std::max<size_t>(1, b->tx.size()) * sizeof(MomPartial)));
-    SCARL_TRY(dmalloc((void**)&b->d_field_mom, std::max<size_t>(1, b->fields.size()) * sizeof(MomPartial)));
+    SCARL_TRY(dmalloc((void**)&b->d_field_mom, std::max<size_t>(1, b->fields.size()) * kMomChunks * sizeof(MomPartial)));
     b->nslots = (int)slot_count.size();
     SCARL_TRY(upload_vec(slot_count, (void**)&b->d_slot_count));
     SCARL_TRY(dmalloc((void**)&b->d_slot_out, std::max<size_t>(1, slot_count.size()) * 5 * sizeof(double)));
