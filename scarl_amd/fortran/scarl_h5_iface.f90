@@ -56,6 +56,39 @@ module scarl_h5_iface
             real(c_double), dimension(*), intent(in) :: v
             integer(c_int32_t), value :: n
         end function
+        integer(c_int) function scarl_h5_dataset_info(h, name, dims, prec) bind(C, name="scarl_h5_dataset_info")
+            import :: c_int, c_int32_t, c_char, c_ptr
+            type(c_ptr), value :: h
+            character(kind=c_char), dimension(*), intent(in) :: name
+            integer(c_int32_t), dimension(3), intent(out) :: dims
+            integer(c_int32_t), intent(out) :: prec
+        end function
+        integer(c_int) function scarl_h5_read_block(h, name, data, prec, ld1, ld2, offset, count) &
+                bind(C, name="scarl_h5_read_block")
+            import :: c_int, c_int32_t, c_int64_t, c_char, c_ptr
+            type(c_ptr), value :: h
+            character(kind=c_char), dimension(*), intent(in) :: name
+            type(c_ptr), value :: data
+            integer(c_int32_t), value :: prec
+            integer(c_int64_t), value :: ld1, ld2
+            integer(c_int32_t), dimension(3), intent(in) :: offset, count
+        end function
+        integer(c_int) function scarl_h5_read_attr_i32(h, object, name, v, cap, n) bind(C, name="scarl_h5_read_attr_i32")
+            import :: c_int, c_int32_t, c_char, c_ptr
+            type(c_ptr), value :: h
+            character(kind=c_char), dimension(*), intent(in) :: object, name
+            integer(c_int32_t), dimension(*), intent(out) :: v
+            integer(c_int32_t), value :: cap
+            integer(c_int32_t), intent(out) :: n
+        end function
+        integer(c_int) function scarl_h5_read_attr_f64(h, object, name, v, cap, n) bind(C, name="scarl_h5_read_attr_f64")
+            import :: c_int, c_int32_t, c_double, c_char, c_ptr
+            type(c_ptr), value :: h
+            character(kind=c_char), dimension(*), intent(in) :: object, name
+            real(c_double), dimension(*), intent(out) :: v
+            integer(c_int32_t), value :: cap
+            integer(c_int32_t), intent(out) :: n
+        end function
         integer(c_int) function scarl_h5_write_times(h, labels, times, ntimes, nprocs) &
                 bind(C, name="scarl_h5_write_times")
             import :: c_int, c_int32_t, c_double, c_char, c_ptr

@@ -65,6 +65,34 @@ module scarl_hip_iface
             integer(c_int32_t), intent(inout) :: seed
             type(c_ptr), value :: x
         end function
+        ! Stat_ScaRL moments and corrL estimator (include/scarl_hip.h)
+        integer(c_int) function scarl_hip_moments(rf, n, nmc, evnt_avg, evnt_std, g_avg, g_std) &
+                bind(C, name="scarl_hip_moments")
+            import :: c_int, c_int32_t, c_int64_t, c_double, c_ptr
+            type(c_ptr), value :: rf
+            integer(c_int64_t), value :: n
+            integer(c_int32_t), value :: nmc
+            real(c_double), dimension(*), intent(out) :: evnt_avg, evnt_std
+            real(c_double), intent(out) :: g_avg, g_std
+        end function
+        integer(c_int) function scarl_hip_rebuild_sk_fft(field, xNStep, dir, nDim, avg, stdDev, sk_part) &
+                bind(C, name="scarl_hip_rebuild_sk_fft")
+            import :: c_int, c_int32_t, c_double, c_ptr
+            type(c_ptr), value :: field
+            integer(c_int32_t), dimension(3), intent(in) :: xNStep
+            integer(c_int32_t), value :: dir, nDim
+            real(c_double), value :: avg, stdDev
+            real(c_double), dimension(*), intent(out) :: sk_part
+        end function
+        integer(c_int) function scarl_hip_rebuild_corrl(sk_tot, nhalf, nDim, xStep, corrL) &
+                bind(C, name="scarl_hip_rebuild_corrl")
+            import :: c_int, c_int32_t, c_double
+            real(c_double), dimension(*), intent(in) :: sk_tot
+            integer(c_int32_t), dimension(3), intent(in) :: nhalf
+            integer(c_int32_t), value :: nDim
+            real(c_double), dimension(3), intent(in) :: xStep
+            real(c_double), dimension(*), intent(out) :: corrL
+        end function
         ! batched Hermitian generator (include/scarl_hip.h)
         integer(c_int) function scarl_hip_batch_create(fields, count, precision, h) bind(C, name="scarl_hip_batch_create")
             import :: c_int, c_int32_t, c_ptr, scarl_field_desc

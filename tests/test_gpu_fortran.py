@@ -101,3 +101,34 @@ def test_fortran_program_matches_python_driver(hip, tmp_path, deck, nb, subst, c
             la, ta = fa.read_times()
             lb, tb = fb.read_times()
             assert la == lb and ta.shape == tb.shape
+
+
+STAT_PROGRAM = os.path.join(ROOT, "scarl_amd", "fortran", "_build", "scarl_stat")
+
+
+@pytest.mark.skipif(not os.path.exists(STAT_PROGRAM), reason="Fortran statistics program not built")
+def test_fortran_stat_program_matches_python(hip, tmp_path):
+    """scarl_stat (Fortran) and main_Stat.main (Python) on copies of one written sample: the
+    same statistics on the sample's root and the same STAT_ file."""
+    import io
+    import shutil
+    from scarl_amd import main_ScaRL, main_Stat, write_output as W
+    print("case: Fortran statistics program vs Python, 3_TEST sample", flush=True)
+    d = _copy_deck("3_TEST", tmp_path / "gen")
+    main_ScaRL.main(str(d), nb_procs=2)
+    src = d / "SAMPLES" / "Sample_Test_1.h5"
+    for sub in ("f", "p"):
+        (tmp_path / sub).mkdir()
+        shutil.copyfile(src, tmp_path / sub / "Sample_Test_1.h5")
+    fpath, ppath = tmp_path / "f" / "Sample_Test_1.h5", tmp_path / "p" / "Sample_Test_1.h5"
+    r = subprocess.run([STAT_PROGRAM], input=f'1\n1\n0\n"{fpath}"\n"samples"\n', capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0 and "EXIT CODE: OK" in r.stdout, r.stdout + r.stderr
+    main_Stat.main(io.StringIO(f'1\n1\n0\n"{ppath}"\n"samples"\n'))
+    for a, b in ((fpath, ppath), (fpath.parent / "STAT_Sample_Test_1.h5", ppath.parent / "STAT_Sample_Test_1.h5")):
+        with W.H5File(str(a)) as fa, W.H5File(str(b)) as fb:
+            na, nb = sorted(fa.attr_names("")), sorted(fb.attr_names(""))
+            assert na == nb and "corrL_out" in na and "Sk_3" in na
+            for n in na:
+                kind = "i" if n in INT_ATTRS else "d"
+                np.testing.assert_array_equal(fa.read_attr("", n, kind), fb.read_attr("", n, kind), err_msg=n)
