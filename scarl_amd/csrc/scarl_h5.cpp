@@ -14,6 +14,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -37,12 +38,11 @@ int fail(int code, const char* fmt, ...) {
 
 // HDF5's own error stack printing is switched off once; failures come back as codes.
 void quiet() {
-    static bool done = false;
-    if (!done) {
+    static std::once_flag once;
+    std::call_once(once, [] {
         H5open();
         H5Eset_auto2(H5E_DEFAULT, nullptr, nullptr);
-        done = true;
-    }
+    });
 }
 
 // Closes an identifier at scope exit.
