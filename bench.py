@@ -179,7 +179,7 @@ def main():
         traffic_from_counters(args.traffic[0], args.traffic[1], os.path.join(ROOT, "profiles", "traffic.json"))
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
+        print(json.dumps({"error": "bench.py measures one GPU; runs on several GPUs are not supported",
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
