@@ -28,6 +28,12 @@ __device__ __forceinline__ cplx<T> gload(const cplx<T>* p) {
     const SCARL_GLOBAL T* q = (const SCARL_GLOBAL T*)p;
     return {q[0], q[1]};
 }
+// The same load with the non-temporal hint, for data read exactly once (global_load ... nt)
+template <typename T>
+__device__ __forceinline__ cplx<T> gload_nt(const cplx<T>* p) {
+    const SCARL_GLOBAL T* q = (const SCARL_GLOBAL T*)p;
+    return {__builtin_nontemporal_load(q), __builtin_nontemporal_load(q + 1)};
+}
 template <typename T>
 __device__ __forceinline__ void gstore(cplx<T>* p, cplx<T> v) {
     SCARL_GLOBAL T* q = (SCARL_GLOBAL T*)p;

@@ -328,8 +328,9 @@ __global__ __launch_bounds__(kBlock) void k_h_xpass(const FieldH* __restrict__ f
     // the blend epilogue (PoU weights, owner order) is staged through LDS; the plain one stores from
     // registers in the last stage
     if constexpr (!FULL && Fft::kXPair && sizeof(T) == 4) {
-        // every X_k loaded once; X_{M-k} comes from the mirrored lane of the row
-        auto xld = [&](const Ctx& c, int k) -> cplx<T> { return gload(c.X + k); };
+        // every X_k loaded once, with the non-temporal hint (5 % on this pass); X_{M-k} comes from
+        // the mirrored lane of the row
+        auto xld = [&](const Ctx& c, int k) -> cplx<T> { return gload_nt(c.X + k); };
         auto xnyq = [&](const Ctx& c) -> cplx<T> { return gload(c.Xn); };
         auto xcomb = [&](int k, cplx<T> a, cplx<T> bb) -> cplx<T> {
             if (k == 0) {
