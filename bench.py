@@ -130,6 +130,67 @@ def traffic_from_counters(fetch_csv, write_csv, out_json):
     print(json.dumps(out, indent=1))
 
 
+def secondary_rows():
+    """Kernel time of the other rows' entry points on one 256^3 FP64 field (the library's HIP-event
+    timing of each launch; the host-array copies of these drop-in calls are not counted), with
+    GB/s against each kernel's algorithmic bytes, or products per second for the lag rows."""
+    from scarl_amd import generator as Gm
+    from scarl_amd import main_ScaRL, normal_ScaRL, spectra_ScaRL, statistics
+
+    n = 256
+    npts = n ** 3
+    rng = np.random.default_rng(5)
+    field = np.asfortranarray(rng.standard_normal((n, n, n)))
+    rows = {}
+
+    def timed(name, fn, kernels, nbytes=None, products=None):
+        fn()  # scratch allocation and first-touch outside the timed call
+        Gm.timing(True)
+        Gm.timing_reset()
+        try:
+            fn()
+            rep = Gm.timing_report()
+        finally:
+            Gm.timing(False)
+        ms = sum(rep[k][1] for k in kernels if k in rep)
+        r = {"kernels": kernels, "kernel_ms": round(ms, 4)}
+        if nbytes is not None and ms > 0:
+            r["alg_bytes"] = nbytes
+            r["GBps"] = round(nbytes / (ms * 1e-3) / 1e9, 1)
+            r["frac_of_8TBps"] = round(nbytes / (ms * 1e-3) / 8e12, 3)
+        if products is not None and ms > 0:
+            r["Gproducts_per_s"] = round(products / (ms * 1e-3) / 1e9, 1)
+        rows[name] = r
+
+    nl = 1 << 24
+    timed("a4 r8vec_uniform_01 (2^24 values)", lambda: normal_ScaRL.r8vec_uniform_01(nl, 12345), ["lcg_uniform"],
+          nbytes=12 * nl)
+    f1 = field.copy(order="F")
+    timed("a5 apply_UnityPartition_mtx (interior of 3x3x3)",
+          lambda: main_ScaRL.apply_UnityPartition_mtx((n, n, n), (21, 21, 21), 1, (1, 1, 1), (3, 3, 3), f1),
+          ["pou"], nbytes=16 * npts)
+    f2 = field.copy(order="F")
+    timed("a7 multiVariateTransformation (lognormal)",
+          lambda: main_ScaRL.multiVariateTransformation(1.0, 0.3, 2, f2), ["marginal"], nbytes=16 * npts)
+    f3 = field.copy(order="F")
+    timed("a8 normalize_field (one rank)",
+          lambda: main_ScaRL.normalize_field((0, 0, 0), (1, 1, 1), (n, n, n), (0, 0, 0), 0, 0, f3),
+          ["box_sums", "normalize"], nbytes=24 * npts)
+    timed("a9 calculate_average_and_stdVar", lambda: statistics.calculate_average_and_stdVar(field.reshape(-1, 1,
+                                                                                                         order="F")),
+          ["sums"], nbytes=8 * npts)
+    timed("a10 rebuild_Sk_FFT along x (256 periodic lags)",
+          lambda: statistics.rebuild_Sk_FFT(field, (n, n, n), 1, 3, 0.0, 1.0), ["lag"], nbytes=8 * npts,
+          products=n * npts)
+    timed("config 5 lag_covariance (3 axes, 33 lags)", lambda: statistics.lag_covariance(field, 0.0, 33), ["lag"],
+          nbytes=24 * npts, products=3 * 33 * npts)
+    def build_sk():
+        return spectra_ScaRL.build_Sk_mtx(1, (3.0, 3.0, 3.0), (1.0, 1.0, 1.0), (64, 64, 64), (128, 128, 128), 7, 0)
+
+    timed("a3 build_Sk_mtx (Np 64^3, sz_c 128^3)", build_sk, ["build_sk"], nbytes=int(build_sk().nbytes))
+    return rows
+
+
 def deck_program():
     """main_ScaRL.main on the MAX_Test deck with #SIZE# = 115, for 8 ranks, in a scratch folder."""
     import shutil
@@ -171,6 +232,7 @@ def main():
     ap.add_argument("--edge", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-deck", action="store_true", help="skip the deck-program figure")
+    ap.add_argument("--no-secondary", action="store_true", help="skip the other rows' kernel timings")
     ap.add_argument("--probe-only", action="store_true", help="run the copy probe and the headline only")
     ap.add_argument("--traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="write profiles/traffic.json from two rocprofv3 --pmc counter CSVs and exit")
@@ -355,6 +417,14 @@ def main():
                "max_abs": float(np.abs(f32_field0.astype(np.float64) - ref).max()),
                "mfma_rel_l2": float(np.linalg.norm(m0 - ref) / np.linalg.norm(ref))}
 
+    # ---- the other rows' kernels on one 256^3 FP64 field ------------------------------------
+    secondary = None
+    if not args.no_secondary:
+        try:
+            secondary = secondary_rows()
+        except Exception as exc:  # never costs the bench line
+            secondary = {"error": str(exc)}
+
     # ---- the deck-driven program (generation + HDF5/XDMF output) ----------------------------
     deck = None
     if not args.no_deck:
@@ -395,6 +465,7 @@ def main():
         "mfma_variant": mfma,
         "moments_field0": {"mean": float(moments[0, 3]), "std": float(moments[0, 4])},
         "cpu_baseline": cpu,
+        "secondary": secondary,
         "deck_program": deck,
     }
     print(json.dumps(out), flush=True)
