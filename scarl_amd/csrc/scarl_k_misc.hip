@@ -109,16 +109,19 @@ __global__ void k_marginal(double* __restrict__ x, int64_t n, int margi, double 
     }
 }
 
-// apply_UnityPartition_mtx (SRC_F/main_ScaRL.f90:1216-1301): f *= sqrt(prod_d w_d)
+// apply_UnityPartition_mtx (SRC_F/main_ScaRL.f90:1216-1301): f *= sqrt(prod_d w_d). A block
+// walks whole x rows, so the y and z weights are formed once per row and no point needs a 64-bit
+// division; the product keeps the order (w_x w_y) w_z.
 __global__ void k_pou(double* __restrict__ x, int n1, int n2, int n3, int3 lo, int3 hi) {
-    const int64_t tot = (int64_t)n1 * n2 * n3;
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < tot; t += (int64_t)gridDim.x * blockDim.x) {
-        const int i = (int)(t % n1);
-        const int64_t r = t / n1;
+    const int64_t rows = (int64_t)n2 * n3;
+    for (int64_t r = blockIdx.x; r < rows; r += gridDim.x) {
         const int j = (int)(r % n2), k = (int)(r / n2);
-        const double w = pou_sqrt_weight<double>(i, n1, lo.x, hi.x) * pou_sqrt_weight<double>(j, n2, lo.y, hi.y) *
-                         pou_sqrt_weight<double>(k, n3, lo.z, hi.z);
-        x[t] *= sqrt(w);
+        const double wy = pou_sqrt_weight<double>(j, n2, lo.y, hi.y), wz = pou_sqrt_weight<double>(k, n3, lo.z, hi.z);
+        double* row = x + r * n1;
+        for (int i = threadIdx.x; i < n1; i += blockDim.x) {
+            const double w = pou_sqrt_weight<double>(i, n1, lo.x, hi.x) * wy * wz;
+            row[i] *= sqrt(w);
+        }
     }
 }
 
@@ -129,7 +132,21 @@ __global__ __launch_bounds__(kBlock) void k_sums(const double* __restrict__ rf, 
     const int e = blockIdx.y;
     const double* x = rf + (int64_t)e * n;
     double s = 0, c = 0, s2 = 0, c2 = 0;
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    const int64_t G = (int64_t)gridDim.x * kBlock;
+    int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    // four loads in flight, added in the order of the grid-stride visit
+    for (; i + 3 * G < n; i += 4 * G) {
+        const double v0 = x[i], v1 = x[i + G], v2 = x[i + 2 * G], v3 = x[i + 3 * G];
+        nadd(s, c, v0);
+        nadd(s2, c2, v0 * v0);
+        nadd(s, c, v1);
+        nadd(s2, c2, v1 * v1);
+        nadd(s, c, v2);
+        nadd(s2, c2, v2 * v2);
+        nadd(s, c, v3);
+        nadd(s2, c2, v3 * v3);
+    }
+    for (; i < n; i += G) {
         const double v = x[i];
         nadd(s, c, v);
         nadd(s2, c2, v * v);
@@ -260,12 +277,16 @@ __global__ __launch_bounds__(kBlock) void k_box_sums(const double* __restrict__ 
                                                      int b3, double centre, MomPartial* __restrict__ partial) {
     __shared__ double red[4 * kBlock];
     double s = 0, c = 0, s2 = 0, c2 = 0;
-    const int64_t tot = (int64_t)b1 * b2 * b3;
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < tot; i += (int64_t)gridDim.x * kBlock) {
-        const int64_t a = i % b1, r = i / b1, b = r % b2, k = r / b2;
-        const double v = x[a + (int64_t)n1 * (b + (int64_t)n2 * k)] - centre;
-        nadd(s, c, v);
-        nadd(s2, c2, v * v);
+    // a block walks whole x rows of the box: one 64-bit division per row, not per point
+    const int64_t rows = (int64_t)b2 * b3;
+    for (int64_t r = blockIdx.x; r < rows; r += gridDim.x) {
+        const int64_t b = r % b2, k = r / b2;
+        const double* row = x + (int64_t)n1 * (b + (int64_t)n2 * k);
+        for (int a = threadIdx.x; a < b1; a += kBlock) {
+            const double v = row[a] - centre;
+            nadd(s, c, v);
+            nadd(s2, c2, v * v);
+        }
     }
     block_reduce_mom(s, c, s2, c2, red, partial + blockIdx.x);
 }
