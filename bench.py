@@ -175,7 +175,7 @@ def secondary_rows():
     f3 = field.copy(order="F")
     timed("a8 normalize_field (one rank)",
           lambda: main_ScaRL.normalize_field((0, 0, 0), (1, 1, 1), (n, n, n), (0, 0, 0), 0, 0, f3),
-          ["box_sums", "normalize"], nbytes=24 * npts)
+          ["box_sums", "normalize"], nbytes=32 * npts)  # mean pass, centred-square pass, normalise
     timed("a9 calculate_average_and_stdVar", lambda: statistics.calculate_average_and_stdVar(field.reshape(-1, 1,
                                                                                                          order="F")),
           ["sums"], nbytes=8 * npts)
