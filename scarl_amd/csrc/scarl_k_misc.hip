@@ -173,12 +173,18 @@ __device__ __forceinline__ uint64_t powmod_m31(uint64_t base, uint64_t e) {
     return r;
 }
 __global__ void k_lcg_uniform(int32_t seed0, int64_t n, double* __restrict__ r, int32_t* __restrict__ seeds) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        // Schrage's step keeps seed_i == seed0 * 16807^i (mod 2^31-1) for any int32 seed0
-        const int64_t s0 = ((int64_t)seed0 % 2147483647ll + 2147483647ll) % 2147483647ll;
-        const uint64_t s = mulmod_m31((uint64_t)s0, powmod_m31(16807ull, (uint64_t)(i + 1)));
+    const int64_t G = (int64_t)gridDim.x * blockDim.x;
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    // Schrage's step keeps seed_i == seed0 * 16807^(i+1) (mod 2^31-1) for any int32 seed0. A thread
+    // forms its first seed by exponentiation and each later one (i + G) with one product by 16807^G.
+    const int64_t s0 = ((int64_t)seed0 % 2147483647ll + 2147483647ll) % 2147483647ll;
+    uint64_t s = mulmod_m31((uint64_t)s0, powmod_m31(16807ull, (uint64_t)(i + 1)));
+    const uint64_t step = powmod_m31(16807ull, (uint64_t)G);
+    for (; i < n; i += G) {
         r[i] = (double)(int32_t)s * 4.656612875e-10;
         if (seeds) seeds[i] = (int32_t)s;
+        s = mulmod_m31(s, step);
     }
 }
 

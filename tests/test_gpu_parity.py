@@ -99,8 +99,9 @@ def test_gen_std_gauss_fft_errors(hip):
 
 # ---- normal_ScaRL ------------------------------------------------------------------------
 @pytest.mark.parametrize("n,seed", [(1, 12345), (3, 12345), (2, 1), (1000, 777), (100_000, -123456789),
-                                    (5, 2147483646)])
+                                    (5, 2147483646), (3_000_001, 987654321)])
 def test_lcg_bit_exact(hip, n, seed):
+    """n = 3000001 exceeds the kernel's 4096 x 256 threads, so threads step by 16807^G."""
     from scarl_amd import normal_ScaRL
     r, s = normal_ScaRL.r8vec_uniform_01(n, seed)
     ro, so = O.r8vec_uniform_01(n, seed)
