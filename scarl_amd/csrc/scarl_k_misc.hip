@@ -114,13 +114,31 @@ __global__ void k_marginal(double* __restrict__ x, int64_t n, int margi, double 
 // division; the product keeps the order (w_x w_y) w_z.
 __global__ void k_pou(double* __restrict__ x, int n1, int n2, int n3, int3 lo, int3 hi) {
     const int64_t rows = (int64_t)n2 * n3;
+    // rows of up to 4 * blockDim points: a thread visits the same x positions in every row, so their
+    // weights are formed once and kept in registers
+    constexpr int kRegX = 4;
+    const bool cached = n1 <= kRegX * (int)blockDim.x;
+    double wxr[kRegX];
+#pragma unroll
+    for (int m = 0; m < kRegX; ++m) {
+        const int i = threadIdx.x + m * blockDim.x;
+        wxr[m] = (cached && i < n1) ? pou_sqrt_weight<double>(i, n1, lo.x, hi.x) : 0.0;
+    }
     for (int64_t r = blockIdx.x; r < rows; r += gridDim.x) {
         const int j = (int)(r % n2), k = (int)(r / n2);
         const double wy = pou_sqrt_weight<double>(j, n2, lo.y, hi.y), wz = pou_sqrt_weight<double>(k, n3, lo.z, hi.z);
         double* row = x + r * n1;
-        for (int i = threadIdx.x; i < n1; i += blockDim.x) {
-            const double w = pou_sqrt_weight<double>(i, n1, lo.x, hi.x) * wy * wz;
-            row[i] *= sqrt(w);
+        if (cached) {
+#pragma unroll
+            for (int m = 0; m < kRegX; ++m) {
+                const int i = threadIdx.x + m * blockDim.x;
+                if (i < n1) row[i] *= sqrt(wxr[m] * wy * wz);
+            }
+        } else {
+            for (int i = threadIdx.x; i < n1; i += blockDim.x) {
+                const double w = pou_sqrt_weight<double>(i, n1, lo.x, hi.x) * wy * wz;
+                row[i] *= sqrt(w);
+            }
         }
     }
 }

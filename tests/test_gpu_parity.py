@@ -124,14 +124,15 @@ def test_lcg_known_answer(hip):
 # ---- main_ScaRL element routines ---------------------------------------------------------
 @pytest.mark.parametrize("shape", [(2, 1, 1), (2, 2, 1), (2, 2, 2), (3, 3, 3)])
 def test_unity_partition(hip, shape):
+    """x rows of 20 points (x weights held in registers) and of 1100 points (formed per point)."""
     from scarl_amd import main_ScaRL
-    Np, Nov = (20, 17, 12), (5, 4, 3)
     rng = np.random.default_rng(1)
-    for pos in [tuple(p) for p in np.ndindex(*shape)][:5]:
-        f = np.asfortranarray(rng.standard_normal(Np))
-        ref = O.apply_pou(f, Nov, pos, shape)
-        main_ScaRL.apply_UnityPartition_mtx(Np, Nov, 1, pos, shape, f)
-        np.testing.assert_allclose(f, ref, rtol=1e-15, atol=1e-15)
+    for Np, Nov in (((20, 17, 12), (5, 4, 3)), ((1100, 3, 2), (40, 2, 2))):
+        for pos in [tuple(p) for p in np.ndindex(*shape)][:5]:
+            f = np.asfortranarray(rng.standard_normal(Np))
+            ref = O.apply_pou(f, Nov, pos, shape)
+            main_ScaRL.apply_UnityPartition_mtx(Np, Nov, 1, pos, shape, f)
+            np.testing.assert_allclose(f, ref, rtol=1e-15, atol=1e-15)
 
 
 def test_marginal(hip):
