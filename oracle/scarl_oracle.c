@@ -429,6 +429,10 @@ int oracle_gen_std_gauss_fft_mpi(double *out, const int32_t L[3], const double x
 
 /* ------------------------------------------------------------------------------------ */
 /* Gamma (Lanczos, g=7, n=9) and K_nu following the ikv scheme                           */
+/* K_nu follows the routine IKV of Shanjie Zhang and Jianming Jin, "Computation of       */
+/* Special Functions" (Wiley, 1996). IKV is copyrighted by Shanjie Zhang and Jianming    */
+/* Jin, who give permission to incorporate it into a user program provided that the      */
+/* copyright is acknowledged (SRC_F/bessel.f90:13522-13532).                             */
 /* (SRC_F/bessel.f90:13522-13785): K_{v0} by series (x<=9) or asymptotic (x>9), I_{v0}   */
 /* by series (x<=18) or asymptotic, I_{v0+1}/I_{v0} by backward recurrence, K_{v0+1} by  */
 /* the Wronskian, then forward recurrence to K_nu.                                         */

@@ -581,6 +581,12 @@ __device__ inline int start_order2_fp64(double x, int n, int mp) {
     return nn + 10;
 }
 
+// The scheme below follows the routine IKV of Shanjie Zhang and Jianming Jin, "Computation of
+// Special Functions" (Wiley, 1996); IKV is copyrighted by Shanjie Zhang and Jianming Jin, who give
+// permission to incorporate it into a user program provided that the copyright is acknowledged
+// (SRC_F/bessel.f90:13522-13532).  The start orders above (start_order1/2) follow their MSTA1
+// and MSTA2 of the same book.
+//
 // K_nu(x), nu >= 0, x > 0: K_{v0} by series (x <= 9) or asymptotic expansion, I_{v0} by
 // series (x <= 18) or asymptotic, I_{v0+1}/I_{v0} from a backward recurrence, K_{v0+1} from
 // the Wronskian I_v K_{v+1} + I_{v+1} K_v = 1/x, then the forward recurrence to K_nu.

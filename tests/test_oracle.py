@@ -36,10 +36,18 @@ def test_lcg_closed_form_jump():
         np.testing.assert_array_equal(r, np.array(expect, np.float64) * 4.656612875e-10)
 
 
-@pytest.mark.skipif(O.ref_lib() is None, reason="reference sources not compiled (oracle/_ref)")
+def _ref_or_skip():
+    """The reference routines compiled from source, looked up inside the test body (never at
+    collection, so a run of the GPU tests does not load them)."""
+    R = O.ref_lib()
+    if R is None:
+        pytest.skip("reference sources not compiled (oracle/_ref)")
+    return R
+
+
 def test_lcg_matches_reference_build():
     import ctypes as C
-    R = O.ref_lib()
+    R = _ref_or_skip()
     for n, seed in ((1, 5), (2, 777), (7, 1), (10, 123456789), (33, 42)):
         sd = C.c_int(seed)
         rr = np.empty(n)
@@ -208,10 +216,9 @@ def test_kv_closed_forms():
         assert abs(O.kv(1.5, x) - k32) < max(kv_tol(1.5, x), 2e-13) * k32
 
 
-@pytest.mark.skipif(O.ref_lib() is None, reason="reference sources not compiled (oracle/_ref)")
 def test_kv_matches_reference_ikv():
     import ctypes as C
-    R = O.ref_lib()
+    R = _ref_or_skip()
     for nu in (0.1, 0.3, 1.3, 2.5):
         for x in (0.05, 1.0, 4.0, 9.5, 25.0):
             bk = np.zeros(8)
@@ -365,6 +372,57 @@ def test_build_sk_shape_and_zero_planes():
     de = (3.1415926535898 ** 2 / 2) * dk ** 2
     v = np.sqrt(np.exp(-(1 + 1 + 1) * de)) * np.sqrt(8.0) * 2 * np.sqrt(dk ** 3)
     assert abs(abs(sk[1, 1, 1]) - v) < 1e-6 * v
+
+
+def _shino_numpy(Np, xRange, corrL, corrMod, seed, transform=None, crop="first"):
+    """gen_std_gauss_Shino_FFT (SRC_F/fftw_ScaRL.f90:157-255) restated with numpy on the oracle's
+    own (separately pinned) Sk: xRange/corrL (:190); sz_c = 2**ceiling(log(2 Np)/log 2) (:194);
+    sz_Sk = shape(Sk_mtx)/2 (:208); the four quadrant copies with 1-based bounds (:217-233), whose
+    x extent is the left-hand sz_Sk(1); FFTW_BACKWARD (unnormalised, exp(+i), :239-243) as
+    prod(sz_c) * numpy.fft.ifftn; the real part of c_out(1:Np(1),1:Np(2),1:Np(3)) (:246).
+    transform/crop plant the plausible mistakes the oracle must not make."""
+    Np = np.asarray(Np)
+    xr = np.asarray(xRange, float) / np.asarray(corrL, float)
+    szc = (2 ** np.ceil(np.log(2.0 * Np) / np.log(2.0))).astype(int)
+    sk = O.build_sk(corrMod, xr, Np, szc, seed)
+    s = np.array(sk.shape) // 2
+    c = np.zeros(tuple(szc), np.complex128)
+    # Fortran c_in(1:s1, a:b, c:d) is numpy c[0:s1, a-1:b, c-1:d]
+    c[0:s[0], 0:s[1], 0:s[2]] = sk[0:s[0], 0:s[1], 0:s[2]]                                   # (+++)
+    c[0:s[0], szc[1] - s[1]:szc[1], 0:s[2]] = sk[0:s[0], s[1]:, 0:s[2]]                        # (+-+)
+    c[0:s[0], 0:s[1], szc[2] - s[2]:szc[2]] = sk[0:s[0], 0:s[1], s[2]:]                        # (++-)
+    c[0:s[0], szc[1] - s[1]:szc[1], szc[2] - s[2]:szc[2]] = sk[0:s[0], s[1]:, s[2]:]           # (+--)
+    out = transform(c) if transform else np.prod(szc) * np.fft.ifftn(c)
+    if crop == "first":
+        out = out[:Np[0], :Np[1], :Np[2]]
+    else:
+        out = out[-Np[0]:, -Np[1]:, -Np[2]:]
+    return out.real, szc, s
+
+
+# Shapes: (a) the y/z quadrants exactly fill the padded grid (k_max < k_cut on x raises k_cut to
+# k_max on y and z, so 2 sz_Sk = sz_c there); (b) unequal padded extents 64 x 32 x 16 with the z
+# quadrants meeting at the grid's middle; (c) the GPU test's first shape.
+SHINO_CASES = [((12, 10, 9), (2.75, 1.0, 1.0), (1.0, 1.0, 1.0), 5),
+               ((20, 9, 5), (19 / 8, 1.0, 0.5), (1.0, 1.0, 1.0), 11),
+               ((20, 18, 16), (9.5, 8.5, 7.5), (4.0, 4.0, 4.0), 3)]
+
+
+@pytest.mark.parametrize("case", range(len(SHINO_CASES)))
+def test_shino_fft_vs_numpy_ifftn(case):
+    Np, xr, cl, seed = SHINO_CASES[case]
+    ref, szc, s = _shino_numpy(Np, xr, cl, 1, seed)
+    got = O.gen_std_gauss_shino_fft(Np, xr, cl, 1, seed)
+    assert np.linalg.norm(got - ref) <= 1e-12 * np.linalg.norm(ref)
+    if case == 0:
+        assert 2 * s[1] == szc[1] and 2 * s[2] == szc[2]   # quadrants reach each other and the edge
+    if case == 1:
+        assert tuple(szc) == (64, 32, 16) and 2 * s[2] == szc[2]
+    # the planted mistakes are far from the oracle: forward transform, crop at the far corner
+    fwd, _, _ = _shino_numpy(Np, xr, cl, 1, seed, transform=np.fft.fftn)
+    far, _, _ = _shino_numpy(Np, xr, cl, 1, seed, crop="last")
+    for wrong in (fwd, far):
+        assert np.linalg.norm(got - wrong) > 1e-2 * np.linalg.norm(ref)
 
 
 # ---- Stat_ScaRL corrL estimator, per-axis covariance, normalize_field -------------------------
