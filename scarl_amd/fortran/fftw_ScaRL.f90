@@ -5,9 +5,30 @@
 module fftw_ScaRL
     use iso_c_binding
     use constants_ScaRL
+    use normal_ScaRL
+    use spectra_ScaRL
     use scarl_hip_iface
     implicit none
 contains
+
+    ! gen_std_gauss_Shino_FFT (SRC_F/fftw_ScaRL.f90:157-255; the reference driver does not call
+    ! it): build_Sk_mtx on xRange_in/corrL, its four quadrants placed on the
+    ! 2**ceiling(log2(2 Np)) padded grid, a 3D C2C backward transform, and the real part of the
+    ! first Np points into data_real_3D.  pointsPerCorrL is unused, as in the reference.
+    subroutine gen_std_gauss_Shino_FFT(data_real_3D, Np, xRange_in, corrL, pointsPerCorrL, corrMod, seed, rank)
+        integer, dimension(3), intent(in) :: Np
+        double precision, dimension(3), intent(in) :: xRange_in
+        integer, intent(in) :: corrMod
+        double precision, dimension(3), intent(in) :: corrL
+        integer, dimension(3), intent(in) :: pointsPerCorrL
+        integer, intent(in) :: seed
+        integer, intent(in) :: rank
+        double precision, dimension(:,:,:), intent(inout), target, contiguous :: data_real_3D
+        if (any(shape(data_real_3D) /= Np)) stop "gen_std_gauss_Shino_FFT: data_real_3D shape /= Np"
+        call scarl_check(scarl_hip_gen_std_gauss_shino_fft(c_loc(data_real_3D), int(Np, c_int32_t), xRange_in, &
+                         corrL, int(pointsPerCorrL, c_int32_t), int(corrMod, c_int32_t), int(seed, c_int32_t), &
+                         int(rank, c_int32_t)), "gen_std_gauss_Shino_FFT")
+    end subroutine gen_std_gauss_Shino_FFT
 
     ! gen_std_gauss_FFT (SRC_F/fftw_ScaRL.f90:261-418)
     subroutine gen_std_gauss_FFT(data_real_3D, Np, xRange, corrL, corrMod, seed, rank, phi)

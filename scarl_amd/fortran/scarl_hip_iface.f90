@@ -53,6 +53,34 @@ module scarl_hip_iface
             type(c_ptr), value :: phi
             integer(c_int32_t), value :: flags
         end function
+        ! spectra_ScaRL: build_Sk_mtx as a shape query and a fill; the Shinozuka consumer
+        integer(c_int) function scarl_hip_build_sk_query(corrMod, xRange, corrL, Np, sz_c, seed, rank, out_shape) &
+                bind(C, name="scarl_hip_build_sk_query")
+            import :: c_int, c_int32_t, c_double
+            integer(c_int32_t), value :: corrMod
+            real(c_double), dimension(3), intent(in) :: xRange, corrL
+            integer(c_int32_t), dimension(3), intent(in) :: Np, sz_c
+            integer(c_int32_t), value :: seed, rank
+            integer(c_int32_t), dimension(3), intent(out) :: out_shape
+        end function
+        integer(c_int) function scarl_hip_build_sk(corrMod, xRange, corrL, Np, sz_c, seed, rank, sk) &
+                bind(C, name="scarl_hip_build_sk")
+            import :: c_int, c_int32_t, c_double, c_ptr
+            integer(c_int32_t), value :: corrMod
+            real(c_double), dimension(3), intent(in) :: xRange, corrL
+            integer(c_int32_t), dimension(3), intent(in) :: Np, sz_c
+            integer(c_int32_t), value :: seed, rank
+            type(c_ptr), value :: sk
+        end function
+        integer(c_int) function scarl_hip_gen_std_gauss_shino_fft(out, Np, xRange, corrL, pointsPerCorrL, corrMod, &
+                seed, rank) bind(C, name="scarl_hip_gen_std_gauss_shino_fft")
+            import :: c_int, c_int32_t, c_double, c_ptr
+            type(c_ptr), value :: out
+            integer(c_int32_t), dimension(3), intent(in) :: Np
+            real(c_double), dimension(3), intent(in) :: xRange, corrL
+            integer(c_int32_t), dimension(3), intent(in) :: pointsPerCorrL
+            integer(c_int32_t), value :: corrMod, seed, rank
+        end function
         integer(c_int) function scarl_hip_r8vec_uniform_01(n, seed, r) bind(C, name="scarl_hip_r8vec_uniform_01")
             import :: c_int, c_int32_t, c_ptr
             integer(c_int32_t), value :: n

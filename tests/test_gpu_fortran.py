@@ -21,6 +21,9 @@ def test_fortran_dropin_driver(tmp_path):
     out = tmp_path / "field.bin"
     phi.ravel(order="F").tofile(str(out) + ".phi")
     stdin = f"{Np[0]} {Np[1]} {Np[2]} {xr[0]} {xr[1]} {xr[2]} {cl[0]} {cl[1]} {cl[2]} 5 2\n"
+    # spectra_ScaRL / Shinozuka case: unequal padded extents (64, 32, 16) for the consumer
+    NpS, xrS, clS, szc, seedS = (20, 9, 5), (4.75, 2.0, 1.0), (2.0, 2.0, 2.0), (64, 32, 16), 11
+    stdin += " ".join(str(v) for v in (*NpS, *xrS, *clS, *szc, seedS)) + "\n"
     r = subprocess.run([DRIVER, str(out)], input=stdin, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     assert r.stdout.strip().splitlines()[-1].strip() == "EXIT CODE: OK"
@@ -37,6 +40,18 @@ def test_fortran_dropin_driver(tmp_path):
     ro, so = O.r8vec_uniform_01(5, 12345)
     np.testing.assert_array_equal(r5, ro)
     assert seed == so
+    o = 16 * N + 44
+    shp = tuple(int(v) for v in raw[o:o + 12].view(np.int32))
+    o += 12
+    nsk = int(np.prod(shp))
+    sk = raw[o:o + 16 * nsk].view(np.complex128).reshape(shp, order="F")
+    o += 16 * nsk
+    xrc = tuple(a / b for a, b in zip(xrS, clS))
+    ref_sk = O.build_sk(1, xrc, NpS, szc, seedS)
+    assert shp == ref_sk.shape
+    np.testing.assert_allclose(sk, ref_sk, rtol=1e-14, atol=1e-15 * np.abs(ref_sk).max())
+    shino = raw[o:o + 8 * int(np.prod(NpS))].view(np.float64).reshape(NpS, order="F")
+    assert rel_l2(shino, O.gen_std_gauss_shino_fft(NpS, xrS, clS, 1, seedS)) <= 1e-12
 
 
 PROGRAM = os.path.join(ROOT, "scarl_amd", "fortran", "_build", "scarl_main")

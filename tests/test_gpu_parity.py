@@ -394,11 +394,20 @@ def test_blend_x_extent_512(hip):
 
 
 def test_blend_partition_of_unity_variance(hip):
-    """With the same field on every subdomain the blend must give back sqrt(sum w) * f = f where
-    the weights sum to one: check on a Gaussian field via sum of squared weights == 1."""
-    w_lo = O.pou_weights_1d(48, 8, 8, True, False)[:8]
-    w_hi = O.pou_weights_1d(48, 8, 8, False, True)[-8:]
-    np.testing.assert_allclose(w_lo + w_hi, 1.0, atol=1e-15)
+    """The HIP apply_UnityPartition_mtx on fields of ones over 2x1x1, 2x2x1 and 2x2x2 topologies:
+    the squared weights of every owner of a global point sum to one (the blend preserves the
+    variance), as the oracle's brute force does (tests/test_oracle.py)."""
+    from scarl_amd import main_ScaRL
+    Np, Nov = (10, 9, 8), (4, 3, 2)
+    for shape in ((2, 1, 1), (2, 2, 1), (2, 2, 2)):
+        G = np.zeros([(Np[d] - Nov[d]) * shape[d] + Nov[d] for d in range(3)])
+        for pos in np.ndindex(*shape):
+            f = np.ones(Np, order="F")
+            main_ScaRL.apply_UnityPartition_mtx(Np, Nov, 1, pos, shape, f)
+            np.testing.assert_allclose(f, O.apply_pou(np.ones(Np), Nov, pos, shape), rtol=0, atol=1e-15)
+            s = [(Np[d] - Nov[d]) * pos[d] for d in range(3)]
+            G[s[0]:s[0] + Np[0], s[1]:s[1] + Np[1], s[2]:s[2] + Np[2]] += f ** 2
+        np.testing.assert_allclose(G, 1.0, rtol=0, atol=1e-14)
 
 
 # ---- spectra_ScaRL: build_Sk_mtx and the Shinozuka consumer -------------------------------
@@ -413,14 +422,27 @@ def test_build_sk_mtx(hip, corrMod):
         np.testing.assert_allclose(sk, ref, rtol=1e-14, atol=1e-15 * np.abs(ref).max())
 
 
-def test_shino_fft(hip):
+# (Np, xRange, corrL, seed): the y/z quadrants exactly filling a 32^3 padded grid; unequal padded
+# extents 64 x 32 x 16; dx = 0.125 corrL with k_max = 8 > k_cut on a 64 x 64 x 32 grid
+SHINO_GPU_CASES = [((12, 10, 9), (2.75, 1.0, 1.0), (1.0, 1.0, 1.0), 5),
+                   ((20, 9, 5), (19 / 8, 1.0, 0.5), (1.0, 1.0, 1.0), 11),
+                   ((20, 18, 16), (9.5, 8.5, 7.5), (4.0, 4.0, 4.0), 3)]
+
+
+@pytest.mark.parametrize("case", range(len(SHINO_GPU_CASES)))
+def test_shino_fft(hip, case):
+    from scarl_amd import spectra_ScaRL
+    Np, xr, cl, seed = SHINO_GPU_CASES[case]
+    out = np.empty(Np, np.float64, order="F")
+    spectra_ScaRL.gen_std_gauss_Shino_FFT(out, Np, xr, cl, (5, 5, 5), 1, seed, 0)
+    ref = O.gen_std_gauss_shino_fft(Np, xr, cl, 1, seed)
+    assert rel_l2(out, ref) <= FP64_TOL
+
+
+def test_shino_fft_coarse_mesh_is_an_error(hip):
     from scarl_amd import spectra_ScaRL, ScarlError
-    # dx = 0.125 corrL: k_max = 8 > k_cut, the quadrants fit the 64x64x32 padded grid
     Np = (20, 18, 16)
     out = np.empty(Np, np.float64, order="F")
-    spectra_ScaRL.gen_std_gauss_Shino_FFT(out, Np, (9.5, 8.5, 7.5), (4.0, 4.0, 4.0), (5, 5, 5), 1, 3, 0)
-    ref = O.gen_std_gauss_shino_fft(Np, (9.5, 8.5, 7.5), (4.0, 4.0, 4.0), 1, 3)
-    assert rel_l2(out, ref) <= FP64_TOL
     # too coarse a mesh: the reference would write outside c_in; here it is an error
     with pytest.raises(ScarlError):
         spectra_ScaRL.gen_std_gauss_Shino_FFT(out, Np, (19.0, 17.0, 15.0), (4.0, 4.0, 3.0), (5, 5, 5), 1, 3, 0)
