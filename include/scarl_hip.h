@@ -68,6 +68,18 @@ int scarl_hip_gen_std_gauss_fft_mpi(double* h_out, const double xRange[3], const
 int scarl_hip_gen_std_gauss_fft_dev(double* d_out, const int32_t Np[3], const double xRange[3],
                                     const double corrL[3], int32_t corrMod, int32_t seed, int32_t rank,
                                     const double* d_phi, int32_t flags);
+/* Allocation-free form of scarl_hip_gen_std_gauss_fft_dev.  It has no counterpart in the
+ * reference; it is for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments and results as the _dev
+ * call (the same kernels), but every device buffer it needs (plan table, twiddles, the two
+ * REDFT01 workspaces) is carved out of one static device array of 97 MiB held by the library on
+ * each GPU it runs on: fields whose workspaces (n1 m2 m3 + n1 n2 m3 doubles, m = Np/2) fit in
+ * 96 MiB are accepted (256^3 is), larger ones fail with SCARL_E_INVALID before any launch.  It calls no hipMalloc/hipFree.  All calls on one GPU share
+ * that array: calls that overlap in time on one GPU race.  The call returns once its kernels are
+ * queued on the library stream (scarl_hip_get_stream); the array keeps the last call's contents. */
+int scarl_hip_gen_std_gauss_fft_dev_noalloc(double* d_out, const int32_t Np[3], const double xRange[3],
+                                            const double corrL[3], int32_t corrMod, int32_t seed, int32_t rank,
+                                            const double* d_phi, int32_t flags);
 /* marginal fused by flags bit 0 (multiVariateTransformation, SRC_F/main_ScaRL.f90:1355-1395) */
 int scarl_hip_set_marginal(int32_t margiFirst, double avg, double std_dev);
 

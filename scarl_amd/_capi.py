@@ -63,6 +63,7 @@ SIGNATURES = {
     "scarl_hip_gen_std_gauss_fft": [_vp, _vp, _vp, _vp, _i32, _i32, _i32, _vp, _i32],
     "scarl_hip_gen_std_gauss_fft_mpi": [_vp, _vp, _vp, _i32, _i32, _i32, _vp, _vp, _i32, _vp, _i32],
     "scarl_hip_gen_std_gauss_fft_dev": [_vp, _vp, _vp, _vp, _i32, _i32, _i32, _vp, _i32],
+    "scarl_hip_gen_std_gauss_fft_dev_noalloc": [_vp, _vp, _vp, _vp, _i32, _i32, _i32, _vp, _i32],
     "scarl_hip_set_marginal": [_i32, _d, _d],
     "scarl_hip_build_sk_query": [_i32, _vp, _vp, _vp, _vp, _i32, _i32, _vp],
     "scarl_hip_build_sk": [_i32, _vp, _vp, _vp, _vp, _i32, _i32, _vp],

@@ -69,6 +69,10 @@ hipError_t launch_build_sk(const void* params, void* sk, int64_t tot, hipStream_
 hipError_t launch_shino_place(const int* dims9, const void* sk, void* c, hipStream_t s);
 hipError_t launch_c2c_xcrop(const PlanDev* plans, int plan, int T, const void* pool, const void* c, int c1, int c2,
                             int np1, int np2, int np3, double* out, size_t lds, hipStream_t s);
+// static scratch of the allocation-free entry point (scarl_noalloc.hip)
+hipError_t noalloc_setup(int device);
+void* noalloc_scratch(int device, size_t* bytes);
+const char* noalloc_error(int device);
 // host mirror of the kernel parameter block of k_build_sk (scarl_k_r.hip)
 struct SkParamsHost {
     int s1, sz2, sz3;
@@ -154,6 +158,9 @@ static int ensure_init() {
     if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(SCARL_E_HIP, "no HIP device available");
     if (g_ctx.device >= n) return fail(SCARL_E_INVALID, "device %d out of range (%d devices)", g_ctx.device, n);
     HIP_TRY(hipSetDevice(g_ctx.device));
+    // the allocation-free entry point's scratch address on this device; a failure here is kept
+    // and reported by that entry point only
+    (void)noalloc_setup(g_ctx.device);
     HIP_TRY(hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking));
     HIP_TRY(hipStreamCreateWithFlags(&g_ctx.gen_stream, hipStreamNonBlocking));
     g_ctx.ready = true;
@@ -951,23 +958,20 @@ static int mode_r_prepare(const int32_t n[3], const int32_t m[3]) {
     return 0;
 }
 
-// d_out, d_phi device pointers; m = octant support
-static int mode_r_run(double* d_out, const int32_t n[3], const int32_t m[3], const double xRange[3],
-                      const double corrL[3], int32_t corrMod, uint64_t seed, const double* d_phi, int32_t flags) {
+// The FieldR of one call (SRC_F/fftw_ScaRL.f90:300-302 k spacing); plans and workspaces are
+// filled in by the caller.
+static int mode_r_desc(FieldR& f, double* d_out, const int32_t n[3], const int32_t m[3], const double xRange[3],
+                       const double corrL[3], int32_t corrMod, uint64_t seed, const double* d_phi, int32_t flags) {
     if (corrMod != CM_GAUSSIAN)
         return fail(SCARL_E_UNSUPPORTED, "gen_std_gauss_FFT implements corrMod = cm_GAUSSIAN only (got %d)", corrMod);
     for (int a = 0; a < 3; ++a) {
         if (n[a] < 1) return fail(SCARL_E_INVALID, "Np(%d) = %d must be >= 1", a + 1, n[a]);
         if (!(corrL[a] > 0.0)) return fail(SCARL_E_INVALID, "corrL(%d) must be positive", a + 1);
     }
-    SCARL_TRY(mode_r_prepare(n, m));
-    FieldR f;
     memset(&f, 0, sizeof f);
     for (int a = 0; a < 3; ++a) {
         f.n[a] = n[a];
         f.m[a] = m[a];
-        f.plan[a] = g_r.plan[a];
-        // SRC_F/fftw_ScaRL.f90:300-302
         const double delta_x = xRange[a] / (double)(n[a] - 1);
         const double k_max = 1.0 / delta_x;
         f.dk[a] = k_max / (double)(n[a] - 1);
@@ -979,31 +983,103 @@ static int mode_r_run(double* d_out, const int32_t n[3], const int32_t m[3], con
     f.stream = 0;
     f.margi = (flags & 1) ? g_ctx.margi : FOM_NONE;
     SCARL_TRY(marginal_params(f.margi, g_ctx.m_avg, g_ctx.m_std, &f.mu, &f.sd));
-    const int64_t t1 = (int64_t)n[0] * m[1] * m[2], t2 = (int64_t)n[0] * n[1] * m[2];
-    void *p1, *p2;
-    SCARL_TRY(scratch_get("r_tmp1", (size_t)std::max<int64_t>(1, t1) * 8, &p1));
-    SCARL_TRY(scratch_get("r_tmp2", (size_t)std::max<int64_t>(1, t2) * 8, &p2));
-    f.tmp1 = static_cast<double*>(p1);
-    f.tmp2 = static_cast<double*>(p2);
     f.out = d_out;
+    return 0;
+}
+
+// Workspace elements (doubles) of the x and y passes: tmp1 (n1, m2, m3), tmp2 (n1, n2, m3).
+static void mode_r_tmp_elems(const int32_t n[3], const int32_t m[3], int64_t* t1, int64_t* t2) {
+    *t1 = std::max<int64_t>(1, (int64_t)n[0] * m[1] * m[2]);
+    *t2 = std::max<int64_t>(1, (int64_t)n[0] * n[1] * m[2]);
+}
+
+// The three passes x, y, z of a filled FieldR.  host_plans: the plan table (host copy) whose
+// device copy is d_plans; use_timed = 0 launches without events (no event objects are created).
+static int mode_r_passes(const FieldR& f, const PlanDev* host_plans, const PlanDev* d_plans, const void* d_pool,
+                         const size_t lds[3], bool use_timed) {
     // the descriptor travels by value in the kernel arguments (no host buffer outlives the call)
-    const int64_t npen_real[3] = {(int64_t)m[1] * m[2], (int64_t)n[0] * m[2], (int64_t)n[0] * n[1]};
+    const int64_t npen_real[3] = {(int64_t)f.m[1] * f.m[2], (int64_t)f.n[0] * f.m[2], (int64_t)f.n[0] * f.n[1]};
     static const char* names[3] = {"r_xpass", "r_ypass", "r_zpass"};
     for (int a = 0; a < 3; ++a) {
-        const PlanDev& pl = g_r.ps->plans[g_r.plan[a]];
+        const PlanDev& pl = host_plans[f.plan[a]];
         const int64_t ncplx = (npen_real[a] + 1) / 2;
         const int64_t nblocks = (ncplx + pl.T - 1) / pl.T;
         if (nblocks > 0x7fffffff) return fail(SCARL_E_UNSUPPORTED, "grid too large");
-        if (a == 0 && (m[1] == 0 || m[2] == 0)) {
+        if (a == 0 && (f.m[1] == 0 || f.m[2] == 0)) {
             // empty octant: x pass has no pencils; tmp1 content is irrelevant (m2 or m3 = 0)
             continue;
         }
-        SCARL_TRY(timed(names[a], [&] {
-            return launch_r_pass(a, pl.spec, f, g_r.ps->d_plans, g_r.ps->d_pool, (int)nblocks, g_r.lds[a],
-                                 g_ctx.stream);
-        }));
+        auto launch = [&] { return launch_r_pass(a, pl.spec, f, d_plans, d_pool, (int)nblocks, lds[a], g_ctx.stream); };
+        if (use_timed) {
+            SCARL_TRY(timed(names[a], launch));
+        } else {
+            const hipError_t e = launch();
+            if (e != hipSuccess) return fail(SCARL_E_HIP, "launch of %s failed: %s", names[a], hipGetErrorString(e));
+        }
     }
     return 0;
+}
+
+// d_out, d_phi device pointers; m = octant support
+static int mode_r_run(double* d_out, const int32_t n[3], const int32_t m[3], const double xRange[3],
+                      const double corrL[3], int32_t corrMod, uint64_t seed, const double* d_phi, int32_t flags) {
+    FieldR f;
+    SCARL_TRY(mode_r_desc(f, d_out, n, m, xRange, corrL, corrMod, seed, d_phi, flags));
+    SCARL_TRY(mode_r_prepare(n, m));
+    for (int a = 0; a < 3; ++a) f.plan[a] = g_r.plan[a];
+    int64_t t1, t2;
+    mode_r_tmp_elems(n, m, &t1, &t2);
+    void *p1, *p2;
+    SCARL_TRY(scratch_get("r_tmp1", (size_t)t1 * 8, &p1));
+    SCARL_TRY(scratch_get("r_tmp2", (size_t)t2 * 8, &p2));
+    f.tmp1 = static_cast<double*>(p1);
+    f.tmp2 = static_cast<double*>(p2);
+    return mode_r_passes(f, g_r.ps->plans.data(), g_r.ps->d_plans, g_r.ps->d_pool, g_r.lds, true);
+}
+
+// The same passes with every device buffer (plan table, twiddle pool, tmp1, tmp2) carved out of
+// the library's static scratch array (scarl_noalloc.hip, noalloc_scratch()): no device memory is
+// allocated.  The plan table and the pool are copied into the scratch in stream order (after any
+// earlier work on the library stream has read them), then the host waits for that copy, so the
+// host tables may die on return; the passes themselves run asynchronously, as in the _dev call.
+static int mode_r_run_noalloc(double* d_out, const int32_t n[3], const int32_t m[3], const double xRange[3],
+                              const double corrL[3], int32_t corrMod, uint64_t seed, const double* d_phi,
+                              int32_t flags) {
+    size_t cap = 0;
+    char* base = static_cast<char*>(noalloc_scratch(g_ctx.device, &cap));
+    if (!base) return fail(SCARL_E_HIP, "%s", noalloc_error(g_ctx.device));
+    FieldR f;
+    SCARL_TRY(mode_r_desc(f, d_out, n, m, xRange, corrL, corrMod, seed, d_phi, flags));
+    PlanSet ps;  // host side only: upload() is never called, so it owns no device memory
+    ps.fp64 = true;
+    size_t lds[3];
+    for (int a = 0; a < 3; ++a) {
+        f.plan[a] = ps.get(n[a], a == 0 ? LAYOUT_BF : LAYOUT_PF, 2);
+        if (f.plan[a] < 0) return -f.plan[a];
+    }
+    for (int a = 0; a < 3; ++a) {
+        lds[a] = (size_t)ps.plans[f.plan[a]].lds_elems * 16;
+        SCARL_TRY(check_lds(lds[a]));
+    }
+    // layout: [plan table | twiddle pool (FP64 complex) | tmp1 | tmp2], each 256-byte aligned
+    auto up = [](size_t v) { return (v + 255) & ~size_t(255); };
+    std::vector<double> pool(std::max<size_t>(2, ps.pool.size()), 0.0);
+    for (size_t i = 0; i < ps.pool.size(); ++i) pool[i] = (double)ps.pool[i];
+    const size_t plan_bytes = ps.plans.size() * sizeof(PlanDev), pool_bytes = pool.size() * sizeof(double);
+    int64_t t1, t2;
+    mode_r_tmp_elems(n, m, &t1, &t2);
+    const size_t o_pool = up(plan_bytes), o_t1 = o_pool + up(pool_bytes), o_t2 = o_t1 + up((size_t)t1 * 8);
+    const size_t need = o_t2 + up((size_t)t2 * 8);
+    if (need > cap)
+        return fail(SCARL_E_INVALID,
+                    "Np = (%d, %d, %d) needs %zu bytes of scratch; the allocation-free entry point holds %zu "
+                    "(workspaces up to 96 MiB, e.g. 256^3 points)", n[0], n[1], n[2], need, cap);
+    HIP_TRY(hipMemcpyAsync(base, ps.plans.data(), plan_bytes, hipMemcpyHostToDevice, g_ctx.stream));
+    HIP_TRY(hipMemcpyAsync(base + o_pool, pool.data(), pool_bytes, hipMemcpyHostToDevice, g_ctx.stream));
+    HIP_TRY(hipStreamSynchronize(g_ctx.stream));
+    f.tmp1 = reinterpret_cast<double*>(base + o_t1);
+    f.tmp2 = reinterpret_cast<double*>(base + o_t2);
+    return mode_r_passes(f, ps.plans.data(), reinterpret_cast<const PlanDev*>(base), base + o_pool, lds, false);
 }
 
 }  // namespace scarl
@@ -1159,6 +1235,16 @@ int scarl_hip_gen_std_gauss_fft_dev(double* d_out, const int32_t Np[3], const do
     const int32_t m[3] = {Np[0] / 2, Np[1] / 2, Np[2] / 2};
     const uint64_t key = (uint64_t)(int64_t)((int64_t)seed + rank);
     return mode_r_run(d_out, Np, m, xRange, corrL, corrMod, key, d_phi, flags);
+}
+
+int scarl_hip_gen_std_gauss_fft_dev_noalloc(double* d_out, const int32_t Np[3], const double xRange[3],
+                                            const double corrL[3], int32_t corrMod, int32_t seed, int32_t rank,
+                                            const double* d_phi, int32_t flags) {
+    SCARL_TRY(ensure_init());
+    if (!d_out || !Np || !xRange || !corrL) return fail(SCARL_E_INVALID, "null argument");
+    const int32_t m[3] = {Np[0] / 2, Np[1] / 2, Np[2] / 2};
+    const uint64_t key = (uint64_t)(int64_t)((int64_t)seed + rank);
+    return mode_r_run_noalloc(d_out, Np, m, xRange, corrL, corrMod, key, d_phi, flags);
 }
 
 // ---- normal_ScaRL ------------------------------------------------------------------------
