@@ -118,6 +118,12 @@ struct Tile {
 // The FP32 x pass of a 512-wide field (N = 256 = 16 x 16) on the matrix cores: split-precision
 // bf16 MFMA radix-16 stages (scarl_mfma.h), selected by SCARL_TUNE_MFMA_DFT16.
 constexpr int kSpecMfma256 = 64;
+// Runtime-planned FP32 transforms of a length that is a multiple of 16 (at least 32) whose first,
+// radix-16 stage runs as split-precision bf16 MFMA products (GenericMfmaFft in scarl_mfma.h);
+// selected by SCARL_TUNE_MFMA_DFT16 like kSpecMfma256.  The tile holds at most 4096 complex
+// elements (four 16-column groups per wave).
+constexpr int kSpecMfmaGeneric = 65;
+constexpr int kMfmaTileElems = 4096;
 
 // Blocks per field of the tile-partial reduction (k_mom_field); the per-field buffer holds
 // kMomChunks partials per field.

@@ -382,6 +382,10 @@ struct ModeHDispatch {
                                   hipStream_t s) {
         if (ntiles <= 0) return hipSuccess;
         const cplx<T>* p = static_cast<const cplx<T>*>(pool);
+        if (spec == kSpecMfmaGeneric) {
+            if constexpr (PREC == 4) strided<GenericMfmaFft<E1>>(axis, gen, fields, tiles, tile0, ntiles, plans, p, lds, s);
+            return hipGetLastError();
+        }
         switch (spec) {
 #define SCARL_CASE(ID, P, L, N, R0, R1, TPP)                                                             \
     case ID:                                                                                             \
@@ -410,6 +414,10 @@ struct ModeHDispatch {
         }
         if (spec == kSpecMfma256) {
             if constexpr (PREC == 4) xpass<MfmaFft256>(blend, fields, tiles, tile0, ntiles, plans, p, partials, lds, s);
+            return hipGetLastError();
+        }
+        if (spec == kSpecMfmaGeneric) {
+            if constexpr (PREC == 4) xpass<GenericMfmaFft<E1>>(blend, fields, tiles, tile0, ntiles, plans, p, partials, lds, s);
             return hipGetLastError();
         }
         switch (spec) {

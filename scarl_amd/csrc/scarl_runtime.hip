@@ -385,9 +385,40 @@ struct PlanSet {
         SCARL_FIXED_SPECS(SCARL_MATCH)
 #undef SCARL_MATCH
         pl.spec = spec;
-        if (mfma && !fp64 && layout == LAYOUT_BF && kind == 1 && N == 256 && sr0 == 16 && sr1 == 16 && stpp == 16)
+        bool mfma_generic = false;
+        if (mfma && !fp64 && layout == LAYOUT_BF && kind == 1 && N == 256 && sr0 == 16 && sr1 == 16 && stpp == 16) {
             pl.spec = kSpecMfma256;  // same radices, tile and twiddle tables: only the kernel differs
-        if (spec) {
+        } else if (mfma && !fp64 && kind != 2 && N >= 32 && N % 16 == 0 && N <= kMfmaTileElems &&
+                   factor(N / 16, 16, rad)) {
+            // first stage radix 16 on the matrix cores, the rest as planned for N/16 (radices <= 16)
+            mfma_generic = true;
+            spec = 0;
+            pl.spec = kSpecMfmaGeneric;
+            rad.insert(rad.begin(), 16);
+        }
+        if (mfma_generic) {
+            pl.nst = (int)rad.size();
+            int Ns = 1, tpp = 1;
+            for (int st = 0; st < pl.nst; ++st) {
+                const int R = rad[st];
+                pl.R[st] = R;
+                pl.Ns[st] = Ns;
+                pl.tw_off[st] = 0;
+                if (Ns > 1) {
+                    pl.tw_off[st] = (int)(pool.size() / 2);
+                    for (int k = 0; k < Ns; ++k)
+                        for (int r = 1; r < R; ++r) {
+                            long double c, sn;
+                            cis_ld((long long)k * r, (long long)Ns * R, c, sn);
+                            pool_push(c, sn);
+                        }
+                }
+                // the MFMA stage uses every lane of the tile; the register stages need tpp threads
+                if (st > 0) tpp = std::max(tpp, (N / R + std::max(1, 16 / R) - 1) / std::max(1, 16 / R));
+                Ns *= R;
+            }
+            pl.tpp = tpp;
+        } else if (spec) {
             pl.nst = sr1 > 1 ? 2 : 1;
             pl.R[0] = sr0;
             pl.Ns[0] = 1;
@@ -446,6 +477,7 @@ struct PlanSet {
         const size_t per = (size_t)(layout == LAYOUT_PF ? N : pl.ldp) * csz;
         const int Tmax = (int)std::max<size_t>(1, (64 * 1024) / per);
         if (!spec) pl.T = std::min(pl.T, Tmax);
+        if (mfma_generic) pl.T = std::max(1, std::min(pl.T, kMfmaTileElems / N));
         pl.lds_elems = (int)(pl.T * (layout == LAYOUT_PF ? N : pl.ldp));
         if (kind == 1) {
             pl.pre_off = (int)(pool.size() / 2);

@@ -615,23 +615,33 @@ def test_config4_blend_1024x1024x256(hip):
     assert abs(mom[3] - 1.0) < 0.07 and abs(mom[4] / mom[3] - 0.5) < 0.1
 
 
-def test_config5_ensemble_256(hip):
+@pytest.mark.parametrize("mfma", [False, True], ids=["stockham", "mfma_bf16x3"])
+def test_config5_ensemble_256(hip, mfma):
     """256 realizations, edges drawn by seed from {64, 96, 128, 160, 192}, half Gaussian (L = 12) and
     half exponential (L = 20), standard-normal marginal, FP32, one batch: per-axis covariance at lags
-    0-32 in the confidence band of each model; two realizations against the FP64 oracle (field and
-    lag covariance)."""
-    from scarl_amd import generator
+    0-32 in the confidence band of each model; two realizations of each of the smallest and largest
+    edges against the FP64 oracle (field and lag covariance).  mfma: every radix-16 first stage (all
+    five edges on y and z, the x half-lengths 32-96) as split-precision bf16 MFMA products."""
+    from scarl_amd import _capi as A, generator
     rng = np.random.default_rng(5)
     edges = rng.choice([64, 96, 128, 160, 192], size=256)
     specs = [generator.FieldSpec(n=(int(e),) * 3, model=1 if i < 128 else 3, corrL=(12.0,) * 3 if i < 128 else (20.0,) * 3,
                                  seed=55, stream=i, margi=1, avg=0.0, std_dev=1.0) for i, e in enumerate(edges)]
-    with generator.Batch.ensemble(specs, 32) as b:
+    _say(f"config 5 ensemble, mfma={mfma}")
+    generator.set_tuning(A.TUNE_MFMA_DFT16, int(mfma))
+    try:
+        b = generator.Batch.ensemble(specs, 32)
+    finally:
+        generator.set_tuning(A.TUNE_MFMA_DFT16, 0)
+    with b:
         b.run(use_graph=True)
         b.sync()
         cov = b.lag_covariance(33)
         mom = b.moments()
-        small = [i for i in range(256) if edges[i] == 64]
-        picks = [small[0], [i for i in small if i >= 128][0]]
+        picks = []
+        for e in (64, 192):
+            sel = [i for i in range(256) if edges[i] == e]
+            picks += [sel[0], [i for i in sel if i >= 128][0]]
         outs = {i: b.output(i) for i in picks}
     assert mom[:, 2].sum() == float(np.sum(edges.astype(np.float64) ** 3))
     stats = {}  # (edge, model) -> exact expectation and one-axis variance
@@ -652,6 +662,55 @@ def test_config5_ensemble_256(hip):
         assert rel_l2(outs[i], ref) <= FP32_TOL
         rc = O.lag_covariance(outs[i].astype(np.float64), mom[i, 3], 33)
         np.testing.assert_allclose(cov[i], rc, rtol=0, atol=1e-10 * np.abs(rc).max())
+
+
+# FP32 shapes whose y/z lengths and x half-lengths are multiples of 16 (32 ... 192): every first
+# stage runs on the matrix cores with SCARL_TUNE_MFMA_DFT16; one 2D field (the y pass generates)
+# and one with host-exported noise
+MFMA_SHAPES = [((64, 96, 128), 1, None), ((96, 160, 64), 3, None), ((128, 192, 32), 2, None),
+               ((160, 64, 96), 1, "host"), ((192, 128, 48), 3, None), ((96, 176, 1), 1, None)]
+
+
+@pytest.mark.parametrize("case", range(len(MFMA_SHAPES)))
+def test_mfma_generic_radix16_stage(hip, case):
+    """The split-precision MFMA radix-16 first stage inside runtime plans of any length that is a
+    multiple of 16: against the FP64 oracle (FP32 bound) and the FP32 Stockham field."""
+    from scarl_amd import _capi as A, generator
+    n, model, noise = MFMA_SHAPES[case]
+    _say(f"mfma generic stage n={n} model={model} noise={noise or 'device'}")
+    L = (10.0, 14.0, 8.0)
+    spec = generator.FieldSpec(n=n, model=model, corrL=L, nu=0.3, seed=21, stream=case, margi=2, avg=1.0,
+                               std_dev=0.3)
+    W = O.noise_hermitian(n, 21, case) if noise else None
+    ref = O.gen_mode_h(n, (1, 1, 1), model, L, 0.3, seed=21, stream=case, W=W, margi=2, avg=1.0, std=0.3)
+    Wc = None if W is None else W.astype(np.complex64)
+    stock = generator.gen_field_h(spec, 32, Wc)
+    generator.set_tuning(A.TUNE_MFMA_DFT16, 1)
+    try:
+        mf = generator.gen_field_h(spec, 32, Wc)
+    finally:
+        generator.set_tuning(A.TUNE_MFMA_DFT16, 0)
+    e_ref, e_st = rel_l2(mf, ref), rel_l2(mf, stock)
+    _say(f"  mfma vs fp64 oracle {e_ref:.2e} (stockham {rel_l2(stock, ref):.2e}), vs stockham {e_st:.2e}")
+    assert e_ref <= FP32_TOL and e_st <= FP32_TOL
+
+
+def test_mfma_generic_blend(hip):
+    """Blended subdomains (x extents 96 and 128, y 64, z 32) with the MFMA first stages: the blend
+    epilogue after the MFMA x pass equals the oracle's gather-sum within the FP32 bound."""
+    from scarl_amd import _capi as A, generator
+    starts, exts = [[0, 80], [0, 48], [0]], [[96, 128], [64, 64], [32]]
+    gdim = (208, 112, 32)
+    spec = generator.FieldSpec(n=(2, 2, 2), model=3, corrL=(12.0, 9.0, 6.0), seed=4, margi=2, avg=1.0, std_dev=0.5)
+    generator.set_tuning(A.TUNE_MFMA_DFT16, 1)
+    try:
+        b = generator.Batch.blend(gdim, starts, exts, spec, 32)
+    finally:
+        generator.set_tuning(A.TUNE_MFMA_DFT16, 0)
+    with b:
+        b.run()
+        G = b.output(-1)
+    assert rel_l2(G, _blend_reference(gdim, starts, exts, spec)) <= FP32_TOL
 
 
 def test_deck_geometry_blend(hip):
