@@ -45,10 +45,16 @@ int scarl_hip_timing_reset(void);
 int scarl_hip_get_stream(void** stream);
 /* plan tuning for batches created afterwards (no counterpart in the reference; the autotuner's
  * knob).  SCARL_TUNE_FP32_ELEMS: complex elements a thread holds per FFT stage, 16 or 32.
- * SCARL_TUNE_MFMA_DFT16: 1 runs the FP32 x pass of 512-wide fields (two radix-16 stages) as
- * split-precision bf16 matrix products on the MFMA units (three bf16 terms per operand, six
- * products, FP32 accumulation); 0 (default) keeps the FP32 Stockham butterflies. */
-enum { SCARL_TUNE_FP32_ELEMS = 1, SCARL_TUNE_MFMA_DFT16 = 2 };
+ * SCARL_TUNE_MFMA_DFT16: 1 runs every FP32 radix-16 first stage (lengths that are multiples of 16,
+ * 32 to 4096; both stages of the 256-point x pass of 512-wide fields) as split-precision bf16 matrix
+ * products on the MFMA units (three bf16 terms per operand, six products, FP32 accumulation);
+ * 0 (default) keeps the FP32 Stockham butterflies.
+ * SCARL_TUNE_AUTOTUNE: 1 makes batch creation time every plan's candidates (compile-time kernel,
+ * runtime radix splits with 16 or 32 elements per thread in either order, pencils per tile, LDS
+ * skew, MFMA first stage) on the batch's own passes and keep the fastest; picks are cached per
+ * process, and in the file named by the SCARL_TUNE_CACHE environment variable when it is set
+ * (the reference's counterpart is FFTW's planner, SRC_F/fftw_ScaRL.f90:378). */
+enum { SCARL_TUNE_FP32_ELEMS = 1, SCARL_TUNE_MFMA_DFT16 = 2, SCARL_TUNE_AUTOTUNE = 3 };
 int scarl_hip_set_tuning(int32_t key, int32_t value);
 
 /* ---- fftw_ScaRL drop-in ------------------------------------------------------------------ */
@@ -191,6 +197,9 @@ int scarl_hip_batch_moments(scarl_batch* h, double* h_out, int32_t max_slots, in
 int scarl_hip_batch_lag_cov(scarl_batch* h, int32_t nlag, double* h_cov);
 /* amplitude prefactor and the FP64 spectrum sum used for the variance normalisation */
 int scarl_hip_batch_field_info(scarl_batch* h, int32_t field, double* amp0, double* spectrum_sum);
+/* one line per plan of the batch: pass (gen / y / x), length, kernel, radices, pencils per tile,
+ * threads per pencil, LDS pitch and skew, how it was chosen, and the autotuner's timings */
+int scarl_hip_batch_plan_report(scarl_batch* h, char* buf, int32_t len);
 /* bytes of device memory the batch holds */
 int scarl_hip_batch_bytes(scarl_batch* h, int64_t* bytes);
 int scarl_hip_batch_destroy(scarl_batch* h);

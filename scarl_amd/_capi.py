@@ -46,7 +46,7 @@ class FieldDesc(C.Structure):
 
 
 # scarl_hip_set_tuning keys
-TUNE_FP32_ELEMS, TUNE_MFMA_DFT16 = 1, 2
+TUNE_FP32_ELEMS, TUNE_MFMA_DFT16, TUNE_AUTOTUNE = 1, 2, 3
 
 # symbol -> (restype, argtypes); the list is the complete ABI of include/scarl_hip.h
 _vp, _i32, _i64, _d = C.c_void_p, C.c_int32, C.c_int64, C.c_double
@@ -90,6 +90,7 @@ SIGNATURES = {
     "scarl_hip_batch_moments": [_vp, _vp, _i32, _pi32],
     "scarl_hip_batch_lag_cov": [_vp, _i32, _vp],
     "scarl_hip_batch_field_info": [_vp, _i32, _pd, _pd],
+    "scarl_hip_batch_plan_report": [_vp, C.c_char_p, _i32],
     "scarl_hip_batch_bytes": [_vp, C.POINTER(C.c_int64)],
     "scarl_hip_batch_destroy": [_vp],
     "scarl_hip_gen_field_h": [_vp, C.POINTER(FieldDesc), _i32, _vp],

@@ -33,7 +33,7 @@ template <typename T, int LAYOUT, bool SIN, bool SOUT, int EC, class Prep, class
 __device__ __forceinline__ void pencil_fft(const PlanDev& pl, const cplx<T>* __restrict__ pool, cplx<T>* lds,
                                            int npen, Prep&& prep, GLoad&& gld, GStore&& gst, GIn&& gin = GIn()) {
     const int tid = threadIdx.x;
-    const int Tt = pl.T, tpp = pl.tpp, N = pl.N, ldp = pl.ldp;
+    const int Tt = pl.T, tpp = pl.tpp, N = pl.N, ldp = pl.ldp, sk = pl.skew;
     int p, b;
     if constexpr (LAYOUT == LAYOUT_PF) {
         p = tid % Tt;
@@ -45,7 +45,7 @@ __device__ __forceinline__ void pencil_fft(const PlanDev& pl, const cplx<T>* __r
     const bool active = (p < npen) && (p < Tt) && (b < tpp);
     auto addr = [&](int n) -> int {
         if constexpr (LAYOUT == LAYOUT_PF) return n * Tt + p;
-        else return p * ldp + n + (n >> 4);
+        else return p * ldp + n + (n >> 4) * sk;
     };
     const auto ctx = prep(active ? p : 0);
     auto ld_g = [&](int n) -> cplx<T> { return gld(ctx, n); };

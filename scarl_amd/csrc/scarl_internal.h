@@ -26,6 +26,7 @@ struct PlanDev {
     int lds_elems;         // complex elements of LDS the tile needs
     int ecap;              // complex elements a thread holds per stage (generic kernel instantiation)
     int spec;              // compile-time planned kernel id (SCARL_FIXED_SPECS), 0 = generic
+    int skew;              // LAYOUT_BF runtime plans: 1 = one LDS element of skew per 16 (ldp includes it)
 };
 
 enum SourceMode { SRC_WORKSPACE = 0, SRC_DEVICE_NOISE = 1, SRC_HOST_NOISE = 2 };

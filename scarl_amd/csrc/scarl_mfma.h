@@ -139,7 +139,7 @@ struct GenericMfmaFft {
                                                int npen, Prep&& prep, GLoad&& gld, GStore&& gst, GIn&& gin = GIn()) {
         static_assert(sizeof(T) == 4, "the MFMA stage serves FP32 plans");
         const int tid = threadIdx.x;
-        const int Tt = pl.T, tpp = pl.tpp, N = pl.N, ldp = pl.ldp;
+        const int Tt = pl.T, tpp = pl.tpp, N = pl.N, ldp = pl.ldp, sk = pl.skew;
         int p, b;
         if constexpr (LAYOUT == LAYOUT_PF) {
             p = tid % Tt;
@@ -151,7 +151,7 @@ struct GenericMfmaFft {
         const bool active = (p < npen) && (p < Tt) && (b < tpp);
         auto addr = [&](int pp, int n) -> int {
             if constexpr (LAYOUT == LAYOUT_PF) return n * Tt + pp;
-            else return pp * ldp + n + (n >> 4);
+            else return pp * ldp + n + (n >> 4) * sk;
         };
         const auto ctx = prep(active ? p : 0);
         auto ld_l = [&](int n) -> cplx<T> { return lds[addr(p, n)]; };

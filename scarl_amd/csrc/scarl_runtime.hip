@@ -131,7 +131,8 @@ struct Context {
     int margi = 0;
     double m_avg = 0.0, m_std = 1.0;
     int fp32_ecap = 16;  // register budget (complex elements per thread) of new FP32 plans
-    bool mfma_dft16 = false;  // FP32 256-point x passes on the matrix cores (new batches)
+    bool mfma_dft16 = false;  // FP32 radix-16 first stages on the matrix cores (new batches)
+    bool autotune = false;    // time every plan's candidates at batch creation
     void* stage[2] = {nullptr, nullptr};         // pinned staging buffers for caller arrays
     hipEvent_t stage_ev[2] = {nullptr, nullptr};  // last DMA that used each staging buffer
 };
@@ -312,15 +313,57 @@ static void cis_ld(long long num, long long den, long double& c, long double& s)
     s = sinl(a);
 }
 
+// One way to plan a transform length (the autotuner's search space).  kind 0 reproduces the
+// fixed rules: the compile-time planned kernel of SCARL_FIXED_SPECS when one matches (or the MFMA
+// kernels when the set asks for them), else the runtime plan with the set's register budget.
+enum PlanVariant { PV_DEFAULT = 0, PV_FIXED = 1, PV_GENERIC = 2, PV_MFMA = 3 };
+struct PlanChoice {
+    int variant = PV_DEFAULT;
+    int spec = 0;   // PV_FIXED: SCARL_FIXED_SPECS id
+    int ecap = 0;   // PV_GENERIC: complex elements per thread and stage (16 or 32; 0 = the set's)
+    int order = 0;  // PV_GENERIC: 0 = largest radix first, 1 = smallest first
+    int tdiv = 1;   // PV_GENERIC / PV_MFMA: pencils per tile divided by this
+    int skew = 1;   // PV_GENERIC / PV_MFMA, butterfly-fastest layout: one LDS element of skew per 16
+};
+
+static std::string choice_name(const PlanChoice& c) {
+    char b[96];
+    switch (c.variant) {
+    case PV_FIXED: snprintf(b, sizeof b, "fixed:%d", c.spec); break;
+    case PV_GENERIC: snprintf(b, sizeof b, "generic:e%d:o%d:t%d:s%d", c.ecap, c.order, c.tdiv, c.skew); break;
+    case PV_MFMA: snprintf(b, sizeof b, "mfma:t%d:s%d", c.tdiv, c.skew); break;
+    default: snprintf(b, sizeof b, "default"); break;
+    }
+    return b;
+}
+
+static bool parse_choice(const char* txt, PlanChoice& c) {
+    c = PlanChoice();
+    if (strcmp(txt, "default") == 0) return true;
+    if (sscanf(txt, "fixed:%d", &c.spec) == 1) { c.variant = PV_FIXED; return true; }
+    if (sscanf(txt, "generic:e%d:o%d:t%d:s%d", &c.ecap, &c.order, &c.tdiv, &c.skew) == 4) {
+        c.variant = PV_GENERIC;
+        return (c.ecap == 16 || c.ecap == 32) && c.tdiv >= 1 && c.tdiv <= 8;
+    }
+    if (sscanf(txt, "mfma:t%d:s%d", &c.tdiv, &c.skew) == 2) {
+        c.variant = PV_MFMA;
+        return c.tdiv >= 1 && c.tdiv <= 8;
+    }
+    return false;
+}
+
 struct PlanSet {
     std::vector<PlanDev> plans;
     std::vector<long double> pool;  // interleaved re/im, long double; rounded once at upload
     std::map<std::string, int> index;
+    std::vector<std::string> keys;     // key of every plan (index order)
+    std::vector<PlanChoice> choices;   // how every plan was made
+    std::map<std::string, PlanChoice> overrides;  // key -> choice (the autotuner's picks)
     PlanDev* d_plans = nullptr;
     void* d_pool = nullptr;
     bool fp64 = false;
-    bool mfma = false;   // plan the FP32 256-point x pass on the matrix cores
-    int ecap = kEcap64;  // register budget of every plan of the set (matches a kernel instantiation)
+    bool mfma = false;   // plan the FP32 x/y/z passes with MFMA radix-16 stages where the length allows
+    int ecap = kEcap64;  // register budget of the default runtime plans (matches a kernel instantiation)
     size_t max_lds = 0;
 
     ~PlanSet() {
@@ -358,67 +401,85 @@ struct PlanSet {
         return true;
     }
 
-    // kind: 0 none, 1 C2R post-twiddle (exp(2 pi i k/(2N)), k<N), 2 DCT pre-twiddle (exp(i pi j/(2N)))
-    int get(int N, int layout, int kind) {
-        char key[64];
-        snprintf(key, sizeof key, "%d/%d/%d", N, layout, kind);
-        auto it = index.find(key);
-        if (it != index.end()) return it->second;
+    // the compile-time planned kernel for (N, precision, layout), or 0
+    int fixed_spec(int N, int layout, int* r0, int* r1, int* tp) const {
+        const int prec = fp64 ? 8 : 4;
+        int spec = 0;
+#define SCARL_MATCH(ID, P, L, NN, A0, A1, TP)                                   \
+        if (spec == 0 && P == prec && L == layout && NN == N) {                 \
+            spec = ID;                                                          \
+            *r0 = A0;                                                           \
+            *r1 = A1;                                                           \
+            *tp = TP;                                                           \
+        }
+        SCARL_FIXED_SPECS(SCARL_MATCH)
+#undef SCARL_MATCH
+        return spec;
+    }
+
+    // Stockham stages of radices rad with their twiddle tables; tpp from the register budget ec
+    // (the first stage excluded when it runs on the matrix cores)
+    void stages(PlanDev& pl, const std::vector<int>& rad, int ec, bool mfma_first) {
+        pl.nst = (int)rad.size();
+        int Ns = 1, tpp = 1;
+        for (int s = 0; s < pl.nst; ++s) {
+            const int R = rad[s];
+            pl.R[s] = R;
+            pl.Ns[s] = Ns;
+            pl.tw_off[s] = 0;
+            if (Ns > 1) {
+                pl.tw_off[s] = (int)(pool.size() / 2);
+                for (int k = 0; k < Ns; ++k)
+                    for (int r = 1; r < R; ++r) {
+                        long double c, sn;
+                        cis_ld((long long)k * r, (long long)Ns * R, c, sn);
+                        pool_push(c, sn);
+                    }
+            }
+            if (!(mfma_first && s == 0)) {
+                const int nb = pl.N / R, bmax = std::max(1, ec / R);
+                tpp = std::max(tpp, (nb + bmax - 1) / bmax);
+            }
+            Ns *= R;
+        }
+        pl.tpp = tpp;
+    }
+
+    // Builds the plan of one choice.  kind: 0 none, 1 C2R post-twiddle (exp(2 pi i k/(2N)), k<N),
+    // 2 DCT pre-twiddle (exp(i pi j/(2N))).  Returns 0, or an error code without touching the set
+    // when the choice does not apply to this length (unknown kernel, LDS, tile size).
+    int make(int N, int layout, int kind, const PlanChoice& ch, PlanDev& pl) {
+        const size_t pool0 = pool.size();
+        auto undo = [&](int code) {
+            pool.resize(pool0);
+            return code;
+        };
         const int csz = fp64 ? 16 : 8;
-        PlanDev pl;
         memset(&pl, 0, sizeof pl);
         pl.N = N;
         pl.ecap = ecap;
         pl.direct_off = -1;
         pl.pre_off = -1;
+        pl.skew = 1;
         std::vector<int> rad;
-        // compile-time planned kernel for this length, precision and layout?
-        int spec = 0, sr0 = 0, sr1 = 0, stpp = 0;
-        const int prec = fp64 ? 8 : 4;
-#define SCARL_MATCH(ID, P, L, NN, A0, A1, TP)                                   \
-        if (spec == 0 && P == prec && L == layout && NN == N) {                 \
-            spec = ID;                                                          \
-            sr0 = A0;                                                           \
-            sr1 = A1;                                                           \
-            stpp = TP;                                                          \
+        int sr0 = 0, sr1 = 0, stpp = 0;
+        const int fspec = fixed_spec(N, layout, &sr0, &sr1, &stpp);
+        const bool mfma_ok = !fp64 && kind != 2 && N >= 32 && N % 16 == 0 && N <= kMfmaTileElems;
+        int variant = ch.variant;
+        if (variant == PV_DEFAULT) {
+            if (mfma && mfma_ok && !(fspec && layout == LAYOUT_BF && kind == 1 && N == 256 && sr0 == 16 &&
+                                     sr1 == 16 && stpp == 16))
+                variant = PV_MFMA;
+            else variant = fspec ? PV_FIXED : PV_GENERIC;
         }
-        SCARL_FIXED_SPECS(SCARL_MATCH)
-#undef SCARL_MATCH
-        pl.spec = spec;
-        bool mfma_generic = false;
-        if (mfma && !fp64 && layout == LAYOUT_BF && kind == 1 && N == 256 && sr0 == 16 && sr1 == 16 && stpp == 16) {
-            pl.spec = kSpecMfma256;  // same radices, tile and twiddle tables: only the kernel differs
-        } else if (mfma && !fp64 && kind != 2 && N >= 32 && N % 16 == 0 && N <= kMfmaTileElems &&
-                   factor(N / 16, 16, rad)) {
-            // first stage radix 16 on the matrix cores, the rest as planned for N/16 (radices <= 16)
-            mfma_generic = true;
-            spec = 0;
-            pl.spec = kSpecMfmaGeneric;
-            rad.insert(rad.begin(), 16);
-        }
-        if (mfma_generic) {
-            pl.nst = (int)rad.size();
-            int Ns = 1, tpp = 1;
-            for (int st = 0; st < pl.nst; ++st) {
-                const int R = rad[st];
-                pl.R[st] = R;
-                pl.Ns[st] = Ns;
-                pl.tw_off[st] = 0;
-                if (Ns > 1) {
-                    pl.tw_off[st] = (int)(pool.size() / 2);
-                    for (int k = 0; k < Ns; ++k)
-                        for (int r = 1; r < R; ++r) {
-                            long double c, sn;
-                            cis_ld((long long)k * r, (long long)Ns * R, c, sn);
-                            pool_push(c, sn);
-                        }
-                }
-                // the MFMA stage uses every lane of the tile; the register stages need tpp threads
-                if (st > 0) tpp = std::max(tpp, (N / R + std::max(1, 16 / R) - 1) / std::max(1, 16 / R));
-                Ns *= R;
-            }
-            pl.tpp = tpp;
-        } else if (spec) {
+        int tdiv = ch.variant == PV_DEFAULT ? 1 : ch.tdiv;
+        if (variant == PV_FIXED) {
+            const int spec = ch.variant == PV_FIXED ? ch.spec : fspec;
+            if (spec != fspec || !spec) return SCARL_E_UNSUPPORTED;
+            pl.spec = spec;
+            if (ch.variant == PV_DEFAULT && mfma && !fp64 && layout == LAYOUT_BF && kind == 1 && N == 256 && sr0 == 16 &&
+                sr1 == 16 && stpp == 16)
+                pl.spec = kSpecMfma256;  // same radices, tile and twiddle tables: only the kernel differs
             pl.nst = sr1 > 1 ? 2 : 1;
             pl.R[0] = sr0;
             pl.Ns[0] = 1;
@@ -434,30 +495,26 @@ struct PlanSet {
                     }
             }
             pl.tpp = stpp;
-        } else if (N >= 2 && factor(N, ecap, rad)) {
-            pl.nst = (int)rad.size();
-            int Ns = 1;
-            int tpp = 1;
-            for (int s = 0; s < pl.nst; ++s) {
-                const int R = rad[s];
-                pl.R[s] = R;
-                pl.Ns[s] = Ns;
-                pl.tw_off[s] = 0;
-                if (Ns > 1) {
-                    pl.tw_off[s] = (int)(pool.size() / 2);
-                    for (int k = 0; k < Ns; ++k)
-                        for (int r = 1; r < R; ++r) {
-                            long double c, sn;
-                            cis_ld((long long)k * r, (long long)Ns * R, c, sn);
-                            pool_push(c, sn);
-                        }
-                }
-                const int nb = N / R, bmax = std::max(1, ecap / R);
-                tpp = std::max(tpp, (nb + bmax - 1) / bmax);
-                Ns *= R;
+            tdiv = 1;
+        } else if (variant == PV_MFMA) {
+            if (!mfma_ok || !factor(N / 16, 16, rad)) return undo(SCARL_E_UNSUPPORTED);
+            rad.insert(rad.begin(), 16);
+            pl.spec = kSpecMfmaGeneric;
+            pl.ecap = 16;
+            if (ch.variant == PV_MFMA) pl.skew = ch.skew;
+            stages(pl, rad, 16, true);
+        } else if (N >= 2 && factor(N, ch.variant == PV_GENERIC ? ch.ecap : ecap, rad)) {
+            const int ec = ch.variant == PV_GENERIC ? ch.ecap : ecap;
+            if (fp64 && ec != kEcap64) return undo(SCARL_E_UNSUPPORTED);
+            if (ch.variant == PV_GENERIC) {
+                if (ch.order) std::reverse(rad.begin(), rad.end());
+                pl.skew = ch.skew;
             }
-            pl.tpp = tpp;
+            pl.spec = 0;
+            pl.ecap = ec;
+            stages(pl, rad, ec, false);
         } else {
+            if (ch.variant != PV_DEFAULT && ch.variant != PV_GENERIC) return undo(SCARL_E_UNSUPPORTED);
             pl.nst = 0;  // direct DFT (N == 1 or a prime factor above 13)
             if (N > 4096) return -fail(SCARL_E_UNSUPPORTED, "length %d has a prime factor above 13 and exceeds 4096", N);
             pl.direct_off = (int)(pool.size() / 2);
@@ -468,16 +525,20 @@ struct PlanSet {
             }
             pl.tpp = std::min(N, 64);
         }
-        if (pl.tpp > kBlock) return -fail(SCARL_E_UNSUPPORTED, "transform length %d too large for one tile", N);
+        if (pl.tpp > kBlock) {
+            if (ch.variant != PV_DEFAULT) return undo(SCARL_E_UNSUPPORTED);
+            return -fail(SCARL_E_UNSUPPORTED, "transform length %d too large for one tile", N);
+        }
         pl.T = kBlock / pl.tpp;
-        pl.ldp = N + N / 16 + 1;
+        pl.ldp = N + (pl.skew ? N / 16 : 0) + 1;
         // keep the LDS tile within 64 KiB so two tiles fit on a CU.  Every plan gets an LDS
         // tile, because passes that stage their input or output use it even for one stage.
         // (compile-time planned kernels fix T = 256 / TPP; the spec table respects the bound)
         const size_t per = (size_t)(layout == LAYOUT_PF ? N : pl.ldp) * csz;
         const int Tmax = (int)std::max<size_t>(1, (64 * 1024) / per);
-        if (!spec) pl.T = std::min(pl.T, Tmax);
-        if (mfma_generic) pl.T = std::max(1, std::min(pl.T, kMfmaTileElems / N));
+        if (!pl.spec) pl.T = std::min(pl.T, Tmax);
+        if (pl.spec == kSpecMfmaGeneric) pl.T = std::max(1, std::min(pl.T, kMfmaTileElems / N));
+        if (pl.spec == 0 || pl.spec == kSpecMfmaGeneric) pl.T = std::max(1, pl.T / tdiv);
         pl.lds_elems = (int)(pl.T * (layout == LAYOUT_PF ? N : pl.ldp));
         if (kind == 1) {
             pl.pre_off = (int)(pool.size() / 2);
@@ -494,9 +555,36 @@ struct PlanSet {
                 pool_push(c, sn);
             }
         }
-        const size_t lds = (size_t)pl.lds_elems * csz;
-        max_lds = std::max(max_lds, lds);
+        return 0;
+    }
+
+    static std::string key_of(int N, int layout, int kind, int role) {
+        char key[64];
+        snprintf(key, sizeof key, "%d/%d/%d/%d", N, layout, kind, role);
+        return key;
+    }
+
+    // Plan index for (N, layout, kind, role); role separates passes that share a length but may
+    // prefer different plans (0: strided, not generating; 1: generating; 2: x pass).  The choice
+    // is the autotuner's pick for the key when there is one, else the default rules.
+    int get(int N, int layout, int kind, int role = 0) {
+        const std::string key = key_of(N, layout, kind, role);
+        auto it = index.find(key);
+        if (it != index.end()) return it->second;
+        PlanChoice ch;
+        auto ov = overrides.find(key);
+        if (ov != overrides.end()) ch = ov->second;
+        PlanDev pl;
+        int rc = make(N, layout, kind, ch, pl);
+        if (rc > 0 && ch.variant != PV_DEFAULT) {
+            ch = PlanChoice();  // a cached pick that no longer applies: default rules
+            rc = make(N, layout, kind, ch, pl);
+        }
+        if (rc) return rc < 0 ? rc : -rc;
+        max_lds = std::max(max_lds, (size_t)pl.lds_elems * (fp64 ? 16 : 8));
         plans.push_back(pl);
+        keys.push_back(key);
+        choices.push_back(ch);
         const int id = (int)plans.size() - 1;
         index[key] = id;
         return id;
@@ -597,6 +685,7 @@ struct scarl_batch {
     int64_t bytes = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
+    std::vector<std::string> tune_notes;     // per plan: the autotuner's timings (empty if untuned)
 
     ~scarl_batch() {
         if (gexec) hipGraphExecDestroy(gexec);
@@ -655,12 +744,13 @@ static int build_field(scarl_batch* b, FieldHost& fh) {
     SCARL_TRY(marginal_params(d.margi, d.avg, d.std_dev, &f.mu, &f.sd));
     f.mom_slot = -1;
     // plans
-    const int kz = (dim == 3) ? b->ps.get(d.n[2], LAYOUT_PF, 0) : 0;
+    // role 1: the generating pass (z, or y for a 2D field); role 0: the in-place y pass
+    const int kz = (dim == 3) ? b->ps.get(d.n[2], LAYOUT_PF, 0, 1) : 0;
     if (kz < 0) return -kz;
-    const int ky = b->ps.get(d.n[1], LAYOUT_PF, 0);
+    const int ky = b->ps.get(d.n[1], LAYOUT_PF, 0, dim == 3 ? 0 : 1);
     if (ky < 0) return -ky;
     // x pass: half-length C2R (even n1, with its pre-twiddle table) or a full-length C2C (odd n1)
-    const int kx = odd ? b->ps.get(d.n[0], LAYOUT_BF, 0) : b->ps.get(d.n[0] / 2, LAYOUT_BF, 1);
+    const int kx = odd ? b->ps.get(d.n[0], LAYOUT_BF, 0, 2) : b->ps.get(d.n[0] / 2, LAYOUT_BF, 1, 2);
     if (kx < 0) return -kx;
     f.plan_z = kz;
     f.plan_y = ky;
@@ -830,6 +920,36 @@ static int upload_vec(const V& v, void** d) {
     return 0;
 }
 
+// Device copies of the plan set, the tile tables, the moment partials and the pipeline events;
+// called again whenever the plans change (autotuning), so it releases what it replaces.
+static int upload_tables(scarl_batch* b) {
+    const size_t rsz = b->fp64 ? 8 : 4;
+    SCARL_TRY(b->ps.upload());
+    make_tiles(b);
+    for (hipEvent_t e : b->gev) hipEventDestroy(e);
+    b->gev.clear();
+    if (b->groups.size() > 1)
+        for (size_t g = 0; g <= b->groups.size(); ++g) {
+            hipEvent_t e;
+            HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            b->gev.push_back(e);
+        }
+    for (const PlanDev& pl : b->ps.plans) SCARL_TRY(check_lds((size_t)pl.lds_elems * 2 * rsz));
+    void** tabs[] = {(void**)&b->d_tz, (void**)&b->d_ty, (void**)&b->d_tx, (void**)&b->d_field_xtiles,
+                     (void**)&b->d_partials};
+    for (void** t : tabs)
+        if (*t) {
+            hipFree(*t);
+            *t = nullptr;
+        }
+    SCARL_TRY(upload_vec(b->tz, (void**)&b->d_tz));
+    SCARL_TRY(upload_vec(b->ty, (void**)&b->d_ty));
+    SCARL_TRY(upload_vec(b->tx, (void**)&b->d_tx));
+    SCARL_TRY(upload_vec(b->field_xtiles, (void**)&b->d_field_xtiles));
+    SCARL_TRY(dmalloc((void**)&b->d_partials, std::max<size_t>(1, b->tx.size()) * sizeof(MomPartial)));
+    return 0;
+}
+
 // Allocate workspaces/outputs, tile tables, moments; upload descriptors.
 static int finalize_batch(scarl_batch* b, const std::vector<double>& slot_count) {
     const size_t rsz = b->fp64 ? 8 : 4;
@@ -864,23 +984,10 @@ static int finalize_batch(scarl_batch* b, const std::vector<double>& slot_count)
             oo += fh.out_elems;
         }
     }
-    SCARL_TRY(b->ps.upload());
-    make_tiles(b);
-    if (b->groups.size() > 1)
-        for (size_t g = 0; g <= b->groups.size(); ++g) {
-            hipEvent_t e;
-            HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-            b->gev.push_back(e);
-        }
-    for (const PlanDev& pl : b->ps.plans) SCARL_TRY(check_lds((size_t)pl.lds_elems * 2 * rsz));
+    SCARL_TRY(upload_tables(b));
     std::vector<FieldH> hf;
     for (auto& fh : b->fields) hf.push_back(fh.f);
     SCARL_TRY(upload_vec(hf, (void**)&b->d_fields));
-    SCARL_TRY(upload_vec(b->tz, (void**)&b->d_tz));
-    SCARL_TRY(upload_vec(b->ty, (void**)&b->d_ty));
-    SCARL_TRY(upload_vec(b->tx, (void**)&b->d_tx));
-    SCARL_TRY(upload_vec(b->field_xtiles, (void**)&b->d_field_xtiles));
-    SCARL_TRY(dmalloc((void**)&b->d_partials, std::max<size_t>(1, b->tx.size()) * sizeof(MomPartial)));
     SCARL_TRY(dmalloc((void**)&b->d_field_mom, std::max<size_t>(1, b->fields.size()) * kMomChunks * sizeof(MomPartial)));
     b->nslots = (int)slot_count.size();
     SCARL_TRY(upload_vec(slot_count, (void**)&b->d_slot_count));
@@ -889,15 +996,16 @@ static int finalize_batch(scarl_batch* b, const std::vector<double>& slot_count)
 }
 
 // Passes of one group: z on stream zs (the generating pass of 3D fields), then y and x on s.
+// only_plan >= 0 launches only the tile classes of that plan (the autotuner's timing).
 static int enqueue_group(scarl_batch* b, const scarl_batch::Group& g, hipStream_t zs, hipStream_t s,
-                         hipEvent_t z_done) {
+                         hipEvent_t z_done, int only_plan = -1) {
     const bool f64 = b->fp64;
     const size_t csz = f64 ? 16 : 8;
-    const int ecap = b->ps.ecap;
     for (const auto& cl : g.cz) {
+        if (only_plan >= 0 && cl.plan != only_plan) continue;
         const PlanDev& pl = b->ps.plans[cl.plan];
         SCARL_TRY(timed("h_zpass", [&] {
-            return launch_h_strided(f64, 2, true, pl.spec, ecap, b->d_fields, b->d_tz, cl.t0, cl.t1 - cl.t0,
+            return launch_h_strided(f64, 2, true, pl.spec, pl.ecap, b->d_fields, b->d_tz, cl.t0, cl.t1 - cl.t0,
                                     b->ps.d_plans, b->ps.d_pool, (size_t)pl.lds_elems * csz, zs);
         }));
     }
@@ -906,17 +1014,19 @@ static int enqueue_group(scarl_batch* b, const scarl_batch::Group& g, hipStream_
         HIP_TRY(hipStreamWaitEvent(s, z_done, 0));
     }
     for (const auto& cl : g.cy) {
+        if (only_plan >= 0 && cl.plan != only_plan) continue;
         const PlanDev& pl = b->ps.plans[cl.plan];
         SCARL_TRY(timed("h_ypass", [&] {
-            return launch_h_strided(f64, 1, b->any_2d, pl.spec, ecap, b->d_fields, b->d_ty, cl.t0, cl.t1 - cl.t0,
+            return launch_h_strided(f64, 1, b->any_2d, pl.spec, pl.ecap, b->d_fields, b->d_ty, cl.t0, cl.t1 - cl.t0,
                                     b->ps.d_plans, b->ps.d_pool, (size_t)pl.lds_elems * csz, s);
         }));
     }
     for (const auto& cl : g.cx) {
+        if (only_plan >= 0 && cl.plan != only_plan) continue;
         const PlanDev& pl = b->ps.plans[cl.plan];
         SCARL_TRY(timed("h_xpass", [&] {
             // x-pass plans of odd n1 are plain C2C (no pre-twiddle table)
-            return launch_h_xpass(f64, pl.spec, ecap, b->global, pl.pre_off < 0, b->d_fields, b->d_tx, cl.t0, cl.t1 - cl.t0,
+            return launch_h_xpass(f64, pl.spec, pl.ecap, b->global, pl.pre_off < 0, b->d_fields, b->d_tx, cl.t0, cl.t1 - cl.t0,
                                   b->ps.d_plans, b->ps.d_pool, b->d_partials, (size_t)pl.lds_elems * csz, s);
         }));
     }
@@ -956,6 +1066,222 @@ static void drop_graph(scarl_batch* b) {
     if (b->graph) hipGraphDestroy(b->graph);
     b->gexec = nullptr;
     b->graph = nullptr;
+}
+
+// ------------------------------------------------------------------------------------------
+// autotuner: per plan of a batch (length, layout, kind, role), every candidate of the search
+// space (compile-time kernel, runtime plans with 16 or 32 elements per thread, radix order,
+// pencils per tile, LDS skew, MFMA first stage) is timed on the batch's own launches of that plan
+// with HIP events, and the fastest is kept.  Picks are cached per precision and plan key for the
+// process, and in the file named by SCARL_TUNE_CACHE when it is set (one "key choice" per line).
+// The reference's counterpart is FFTW's planner (SRC_F/fftw_ScaRL.f90:378, FFTW_ESTIMATE).
+// ------------------------------------------------------------------------------------------
+struct TunePick {
+    PlanChoice choice;
+    double ms = 0.0, default_ms = 0.0;
+    int tried = 0;
+};
+static std::map<std::string, TunePick> g_tune_cache;
+static bool g_tune_file_loaded = false;
+
+static void tune_file_load() {
+    if (g_tune_file_loaded) return;
+    g_tune_file_loaded = true;
+    const char* path = getenv("SCARL_TUNE_CACHE");
+    if (!path || !*path) return;
+    FILE* f = fopen(path, "r");
+    if (!f) return;
+    char key[128], ch[128];
+    while (fscanf(f, "%127s %127s", key, ch) == 2) {
+        TunePick tp;
+        if (parse_choice(ch, tp.choice)) g_tune_cache[key] = tp;
+    }
+    fclose(f);
+}
+
+static void tune_file_append(const std::string& key, const PlanChoice& c) {
+    const char* path = getenv("SCARL_TUNE_CACHE");
+    if (!path || !*path) return;
+    FILE* f = fopen(path, "a");
+    if (!f) return;
+    fprintf(f, "%s %s\n", key.c_str(), choice_name(c).c_str());
+    fclose(f);
+}
+
+static std::vector<PlanChoice> plan_candidates(const PlanSet& ps, int N, int layout) {
+    std::vector<PlanChoice> c(1);  // the default rules first
+    int r0, r1, tp;
+    const int fs = ps.fixed_spec(N, layout, &r0, &r1, &tp);
+    if (fs) {
+        PlanChoice f;
+        f.variant = PV_FIXED;
+        f.spec = fs;
+        c.push_back(f);
+    }
+    const std::vector<int> skews = layout == LAYOUT_BF ? std::vector<int>{1, 0} : std::vector<int>{1};
+    for (int ec : ps.fp64 ? std::vector<int>{16} : std::vector<int>{16, 32})
+        for (int order : {0, 1})
+            for (int tdiv : {1, 2})
+                for (int sk : skews) {
+                    PlanChoice g;
+                    g.variant = PV_GENERIC;
+                    g.ecap = ec;
+                    g.order = order;
+                    g.tdiv = tdiv;
+                    g.skew = sk;
+                    c.push_back(g);
+                }
+    if (!ps.fp64)
+        for (int tdiv : {1, 2})
+            for (int sk : skews) {
+                PlanChoice m;
+                m.variant = PV_MFMA;
+                m.tdiv = tdiv;
+                m.skew = sk;
+                c.push_back(m);
+            }
+    return c;
+}
+
+// what distinguishes two plans' kernels and tiles (candidates with the same signature are one)
+static std::string plan_signature(const PlanDev& p) {
+    std::string s = std::to_string(p.spec) + "/" + std::to_string(p.ecap) + "/" + std::to_string(p.T) + "/" +
+                    std::to_string(p.tpp) + "/" + std::to_string(p.ldp) + "/" + std::to_string(p.skew) + "/";
+    for (int i = 0; i < p.nst; ++i) s += std::to_string(p.R[i]) + ",";
+    return s;
+}
+
+// Average time of the batch's launches of one plan (every group), eager on the library stream.
+static int time_plan(scarl_batch* b, int plan, int reps, double* ms) {
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    int rc = 0;
+    for (int r = -1; r < reps && !rc; ++r) {
+        if (r == 0 && hipEventRecord(e0, g_ctx.stream) != hipSuccess) rc = fail(SCARL_E_HIP, "event record");
+        for (const auto& g : b->groups) {
+            if (rc) break;
+            rc = enqueue_group(b, g, g_ctx.stream, g_ctx.stream, nullptr, plan);
+        }
+    }
+    float t = 0.f;
+    if (!rc) {
+        hipError_t e = hipEventRecord(e1, g_ctx.stream);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+        if (e != hipSuccess) rc = fail(SCARL_E_HIP, "autotuner timing: %s", hipGetErrorString(e));
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    *ms = t / reps;
+    return rc;
+}
+
+static std::string plan_line(const PlanSet& ps, int i) {
+    const PlanDev& p = ps.plans[i];
+    int N, layout, kind, role;
+    sscanf(ps.keys[i].c_str(), "%d/%d/%d/%d", &N, &layout, &kind, &role);
+    static const char* roles[3] = {"y", "gen", "x"};
+    char buf[256];
+    std::string rad;
+    for (int s = 0; s < p.nst; ++s) rad += (s ? "x" : "") + std::to_string(p.R[s]);
+    if (p.nst == 0) rad = "direct";
+    const char* kern = p.spec == kSpecMfmaGeneric ? "mfma-generic" : p.spec == kSpecMfma256 ? "mfma-256"
+                                                                    : p.spec ? "fixed" : "generic";
+    snprintf(buf, sizeof buf, "pass=%s N=%d kernel=%s spec=%d radices=%s ecap=%d T=%d tpp=%d ldp=%d skew=%d choice=%s",
+             role >= 0 && role < 3 ? roles[role] : "?", N, kern, p.spec, rad.c_str(), p.ecap, p.T, p.tpp, p.ldp,
+             p.skew, choice_name(ps.choices[i]).c_str());
+    return buf;
+}
+
+static void swap_plansets(PlanSet& a, PlanSet& b) {
+    std::swap(a.plans, b.plans);
+    std::swap(a.pool, b.pool);
+    std::swap(a.index, b.index);
+    std::swap(a.keys, b.keys);
+    std::swap(a.choices, b.choices);
+    std::swap(a.overrides, b.overrides);
+    std::swap(a.d_plans, b.d_plans);
+    std::swap(a.d_pool, b.d_pool);
+    std::swap(a.fp64, b.fp64);
+    std::swap(a.mfma, b.mfma);
+    std::swap(a.ecap, b.ecap);
+    std::swap(a.max_lds, b.max_lds);
+}
+
+static int autotune_batch(scarl_batch* b) {
+    drop_graph(b);
+    tune_file_load();
+    const bool saved_timing = g_ctx.timing;
+    g_ctx.timing = false;
+    struct Restore {
+        bool v;
+        ~Restore() { g_ctx.timing = v; }
+    } restore{saved_timing};
+    // one generation with the default plans, so that every workspace holds field data
+    SCARL_TRY(enqueue_batch(b));
+    HIP_TRY(hipStreamSynchronize(g_ctx.stream));
+    const std::string prec = b->fp64 ? "fp64/" : "fp32/";
+    const int nplans = (int)b->ps.plans.size();
+    b->tune_notes.assign(nplans, "");
+    std::map<std::string, PlanChoice> picks;
+    for (int pi = 0; pi < nplans; ++pi) {
+        const std::string key = b->ps.keys[pi];
+        int N, layout, kind, role;
+        sscanf(key.c_str(), "%d/%d/%d/%d", &N, &layout, &kind, &role);
+        auto hit = g_tune_cache.find(prec + key);
+        if (hit != g_tune_cache.end()) {
+            picks[key] = hit->second.choice;
+            b->tune_notes[pi] = " cached";
+            continue;
+        }
+        const PlanDev keep = b->ps.plans[pi];
+        TunePick best;
+        best.ms = 1e30;
+        std::map<std::string, bool> seen;
+        for (const PlanChoice& c : plan_candidates(b->ps, N, layout)) {
+            PlanDev pd;
+            if (b->ps.make(N, layout, kind, c, pd) != 0) continue;
+            const std::string sig = plan_signature(pd);
+            if (seen.count(sig)) continue;
+            seen[sig] = true;
+            if (check_lds((size_t)pd.lds_elems * (b->fp64 ? 16 : 8)) != 0) continue;
+            b->ps.plans[pi] = pd;
+            SCARL_TRY(upload_tables(b));
+            double ms = 0.0;
+            SCARL_TRY(time_plan(b, pi, 3, &ms));
+            ++best.tried;
+            if (c.variant == PV_DEFAULT) best.default_ms = ms;
+            if (ms < best.ms) {
+                best.ms = ms;
+                best.choice = c;
+            }
+        }
+        b->ps.plans[pi] = keep;
+        if (best.tried == 0) continue;
+        picks[key] = best.choice;
+        g_tune_cache[prec + key] = best;
+        tune_file_append(prec + key, best.choice);
+        char note[128];
+        snprintf(note, sizeof note, " tuned_ms=%.4f default_ms=%.4f candidates=%d", best.ms, best.default_ms,
+                 best.tried);
+        b->tune_notes[pi] = note;
+    }
+    // a compact plan set with the picks, same indices (same keys in the same order)
+    PlanSet fresh;
+    fresh.fp64 = b->ps.fp64;
+    fresh.mfma = b->ps.mfma;
+    fresh.ecap = b->ps.ecap;
+    fresh.overrides = picks;
+    for (int pi = 0; pi < nplans; ++pi) {
+        int N, layout, kind, role;
+        sscanf(b->ps.keys[pi].c_str(), "%d/%d/%d/%d", &N, &layout, &kind, &role);
+        const int id = fresh.get(N, layout, kind, role);
+        if (id != pi) return fail(SCARL_E_HIP, "autotuner: plan %d rebuilt as %d", pi, id < 0 ? -id : id);
+    }
+    swap_plansets(b->ps, fresh);
+    SCARL_TRY(upload_tables(b));
+    return 0;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1207,6 +1533,11 @@ int scarl_hip_set_tuning(int32_t key, int32_t value) {
     if (key == SCARL_TUNE_MFMA_DFT16) {
         if (value != 0 && value != 1) return fail(SCARL_E_INVALID, "SCARL_TUNE_MFMA_DFT16 takes 0 or 1");
         g_ctx.mfma_dft16 = value != 0;
+        return 0;
+    }
+    if (key == SCARL_TUNE_AUTOTUNE) {
+        if (value != 0 && value != 1) return fail(SCARL_E_INVALID, "SCARL_TUNE_AUTOTUNE takes 0 or 1");
+        g_ctx.autotune = value != 0;
         return 0;
     }
     return fail(SCARL_E_INVALID, "unknown tuning key %d", key);
@@ -1679,6 +2010,7 @@ int scarl_hip_batch_create(const scarl_field_desc* fields, int32_t count, int32_
     }
     SCARL_TRY(finish_amplitudes(b.get()));
     SCARL_TRY(finalize_batch(b.get(), slot_count));
+    if (g_ctx.autotune) SCARL_TRY(autotune_batch(b.get()));
     *h = b.release();
     return 0;
 }
@@ -1775,6 +2107,7 @@ int scarl_hip_blend_create(const int32_t gdim[3], const int32_t nb[3], const int
                 f.out_ld2 = (int64_t)gdim[0] * gdim[1];
             }
     SCARL_TRY(upload_fields(b.get()));
+    if (g_ctx.autotune) SCARL_TRY(autotune_batch(b.get()));
     *h = b.release();
     return 0;
 }
@@ -1877,6 +2210,18 @@ int scarl_hip_batch_field_info(scarl_batch* b, int32_t field, double* amp0, doub
 int scarl_hip_batch_bytes(scarl_batch* b, int64_t* bytes) {
     if (!b || !bytes) return fail(SCARL_E_INVALID, "null argument");
     *bytes = b->bytes;
+    return 0;
+}
+
+int scarl_hip_batch_plan_report(scarl_batch* b, char* buf, int32_t len) {
+    if (!b || !buf || len <= 0) return fail(SCARL_E_INVALID, "null argument");
+    std::string s;
+    for (int i = 0; i < (int)b->ps.plans.size(); ++i) {
+        s += plan_line(b->ps, i);
+        if (i < (int)b->tune_notes.size()) s += b->tune_notes[i];
+        s += "\n";
+    }
+    snprintf(buf, (size_t)len, "%s", s.c_str());
     return 0;
 }
 

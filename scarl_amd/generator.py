@@ -146,6 +146,20 @@ class Batch:
         A.call("scarl_hip_batch_field_info", self._h, int(i), C.byref(a), C.byref(s))
         return a.value, s.value
 
+    def plan_report(self):
+        """One dict per plan of the batch (pass, N, kernel, radices, T, tpp, ldp, skew, choice and,
+        when the batch was autotuned, tuned_ms / default_ms / candidates, or cached)."""
+        buf = C.create_string_buffer(1 << 16)
+        A.call("scarl_hip_batch_plan_report", self._h, buf, len(buf))
+        rows = []
+        for line in buf.value.decode().splitlines():
+            row = {}
+            for tok in line.split():
+                k, _, v = tok.partition("=")
+                row[k] = v if v else True
+            rows.append(row)
+        return rows
+
     def nbytes(self):
         b = C.c_int64()
         A.call("scarl_hip_batch_bytes", self._h, C.byref(b))
@@ -234,6 +248,7 @@ def seeded_tiling(gdim, nb, overlap, choices, seed):
 
 def set_tuning(key, value):
     """Plan tuning for batches created afterwards: A.TUNE_FP32_ELEMS (16 or 32 complex elements
-    per thread) or A.TUNE_MFMA_DFT16 (1: the FP32 x pass of 512-wide fields as split-precision
-    bf16 MFMA radix-16 stages; 0: FP32 Stockham butterflies)."""
+    per thread), A.TUNE_MFMA_DFT16 (1: FP32 radix-16 first stages as split-precision bf16 MFMA
+    products; 0: FP32 Stockham butterflies) or A.TUNE_AUTOTUNE (1: time every plan's candidates at
+    batch creation and keep the fastest)."""
     A.call("scarl_hip_set_tuning", int(key), int(value))

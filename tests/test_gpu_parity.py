@@ -4,6 +4,8 @@ Bars: FP64 paths <= 1e-12 relative L2 (reference-parity REDFT01 and Hermitian C2
 integer LCG bit-exact; FP32 Hermitian path <= 1e-5 relative L2 against the FP64 oracle on the
 same white noise.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -711,6 +713,72 @@ def test_mfma_generic_blend(hip):
         b.run()
         G = b.output(-1)
     assert rel_l2(G, _blend_reference(gdim, starts, exts, spec)) <= FP32_TOL
+
+
+def test_autotuned_batches_match_oracle(hip):
+    """Batches created with SCARL_TUNE_AUTOTUNE: every plan reports its pick and timings, the fields
+    equal the untuned batch's within the FP32 bound and the FP64 oracle (FP64 at 1e-12); a second
+    batch of the same shapes reuses the process cache."""
+    from scarl_amd import _capi as A, generator
+    shapes = [(96, 64, 128), (64, 160, 48), (128, 96, 64), (80, 48, 176)]
+    specs = [generator.FieldSpec(n=n, model=1 + i % 3, corrL=(9.0, 7.0, 5.0), nu=0.7, seed=77, stream=i, margi=2,
+                                 avg=1.0, std_dev=0.4) for i, n in enumerate(shapes)]
+    with generator.Batch.ensemble(specs, 32) as b0:
+        b0.run()
+        plain = [b0.output(i) for i in range(len(specs))]
+    generator.set_tuning(A.TUNE_AUTOTUNE, 1)
+    try:
+        bt = generator.Batch.ensemble(specs, 32)
+        b64 = generator.Batch.ensemble(specs[:2], 64)
+        again = generator.Batch.ensemble(specs, 32)
+    finally:
+        generator.set_tuning(A.TUNE_AUTOTUNE, 0)
+    with bt, b64, again:
+        rep = bt.plan_report()
+        _say("  " + "\n  ".join(f"{r['pass']} N={r['N']} {r['kernel']} {r['radices']} T={r['T']} {r['choice']} "
+                                  f"tuned={r.get('tuned_ms')} default={r.get('default_ms')}" for r in rep))
+        assert rep and all("tuned_ms" in r and float(r["tuned_ms"]) <= float(r["default_ms"]) * 1.5 for r in rep)
+        assert all(r.get("cached") for r in again.plan_report())
+        for bb in (bt, again):
+            bb.run(use_graph=True)
+            bb.sync()
+            for i, sp in enumerate(specs):
+                ref = O.gen_mode_h(sp.n, (1, 1, 1), sp.model, sp.corrL, sp.nu, seed=77, stream=i, margi=2, avg=1.0,
+                                   std=0.4)
+                out = bb.output(i)
+                assert rel_l2(out, ref) <= FP32_TOL and rel_l2(out, plain[i]) <= FP32_TOL
+        b64.run()
+        for i, sp in enumerate(specs[:2]):
+            ref = O.gen_mode_h(sp.n, (1, 1, 1), sp.model, sp.corrL, sp.nu, seed=77, stream=i, margi=2, avg=1.0, std=0.4)
+            assert rel_l2(b64.output(i), ref) <= FP64_TOL
+
+
+def test_autotune_cache_file(hip, tmp_path):
+    """SCARL_TUNE_CACHE: a process that tunes writes its picks; a later process reads them and
+    tunes nothing (its plans report "cached" with the same choices)."""
+    import subprocess
+    import sys
+    from conftest import ROOT
+    cache = tmp_path / "tune.txt"
+    code = (
+        "from scarl_amd import _capi as A, generator as G\n"
+        "A.call('scarl_hip_init', 0)\n"
+        "G.set_tuning(A.TUNE_AUTOTUNE, 1)\n"
+        "b = G.Batch.ensemble([G.FieldSpec(n=(64, 96, 80), model=1, corrL=(6.0, 6.0, 6.0), seed=1)], 32)\n"
+        "print(';'.join(r['choice'] + ('/cached' if r.get('cached') else '') for r in b.plan_report()))\n"
+        "b.close()\n")
+    env = dict(os.environ, SCARL_TUNE_CACHE=str(cache))
+    outs = []
+    for _ in range(2):
+        r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, cwd=ROOT, env=env,
+                           timeout=300)
+        assert r.returncode == 0, r.stdout + r.stderr
+        outs.append(r.stdout.strip().splitlines()[-1])
+    lines = cache.read_text().split("\n")
+    assert len([l for l in lines if l.startswith("fp32/")]) == 3
+    first, second = outs[0].split(";"), outs[1].split(";")
+    assert not any(c.endswith("/cached") for c in first)
+    assert [c + "/cached" for c in first] == second
 
 
 def test_deck_geometry_blend(hip):
