@@ -2,16 +2,26 @@
 
 Headline (value): batched FP32 Hermitian generator, 3D lognormal von Karman fields
 (512^3, L = 32 cells, nu = 0.3, lognormal mean 1 CoV 0.3), B realisations per step with device
-Philox white noise, replayed as one hipGraph: Gpoints/s of generated output points.
-A step = one generation of the B fields (spectrum + noise, z/y/x passes, marginal, moments).
+Philox white noise, replayed as one hipGraph, plans picked by the library's autotuner at batch
+creation: Gpoints/s of generated output points.  A step = one generation of the B fields
+(spectrum + noise, z/y/x passes, marginal, moments).
 
 Also reported on the same line:
+  roofline          -- the pass with the most time per step: algorithmic bytes per launch / average
+                       duration (HIP events on the library stream), against 8 TB/s; roofline_passes
+                       gives every pass, roofline_least_efficient the pass furthest from 8 TB/s
+  default_plans     -- the same batch with the planner's fixed rules (no autotuning)
   plain_fp64        -- the same spec, one FP64 field per launch (the plain drop-in path)
-  dropin_redft01    -- the reference-parity REDFT01 drop-in gen_std_gauss_FFT on 256^3 (FP64)
-  roofline          -- dominant kernel: algorithmic bytes per launch / average duration (HIP
-                       events on the library stream), against 8 TB/s
-  cpu_baseline      -- the FP64 oracle (C port, OpenMP) on one field of the same workload; that
-                       field also gives the FP32-vs-FP64 relative L2 error on shared noise.
+  dropin_redft01    -- config 2: the reference-parity REDFT01 drop-in gen_std_gauss_FFT on 256^3
+                       (FP64, lognormal), device-resident, and its allocation-free form
+  config1_2d        -- 2D 256^2 Gaussian FP64 field, checked against the oracle
+  config4_blend     -- 1024 x 1024 x 256 from 8 x 8 x 2 blended subdomains, FP32, one hipGraph
+  config5_ensemble  -- 256 mixed-size realizations, FP32 Stockham vs bf16x3 MFMA radix-16 stages vs
+                       autotuned, plus the per-axis lag covariance at lags 0-32
+  cpu_baseline      -- the oracle's C restatement of the reference FFTW path (REDFT01 FP64 + lognormal
+                       marginal, OpenMP over the host cores) on config 2, beside scipy.fft.dctn
+                       (pocketfft) on the same array
+  fp32_vs_fp64      -- field 0 of the headline against the FP64 oracle on the same noise
   deck_program      -- the deck-driven program end to end on the reference's MAX_Test deck
                        (SIZE 115: 461^3 points, 8 ranks' subdomains in one blend): generation and
                        the HDF5/XDMF output, timed separately.
@@ -223,6 +233,238 @@ def deck_program():
         shutil.rmtree(d, ignore_errors=True)
 
 
+def _batch_bytes(shapes, prec_bytes):
+    """Algorithmic bytes per step of each pass summed over fields of the given extents."""
+    tot = {}
+    for n in shapes:
+        for k, v in pass_bytes(n, prec_bytes).items():
+            tot[k] = tot.get(k, 0) + v
+    return tot
+
+
+def _steps(batch, steps, warmup):
+    """Seconds per step over `steps` graph replays after `warmup` ones."""
+    for _ in range(max(warmup, 1)):
+        batch.run(use_graph=True)
+    batch.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch.run(use_graph=True)
+    batch.sync()
+    return (time.perf_counter() - t0) / steps
+
+
+def _pass_table(batch, nruns, step_bytes):
+    """Per-pass HIP-event durations of eager runs (every launch alone on the library stream) with
+    GB/s and the fraction of 8 TB/s against the algorithmic bytes of each launch."""
+    from scarl_amd import generator as Gm
+    Gm.timing(True)
+    Gm.timing_reset()
+    try:
+        for _ in range(nruns):
+            batch.run(use_graph=False)
+        rep = Gm.timing_report()
+    finally:
+        Gm.timing(False)
+    passes = {}
+    for k, (cnt, tot) in rep.items():
+        avg = tot / cnt
+        per_step = cnt / nruns
+        passes[k] = {"avg_ms": round(avg, 5), "launches_per_step": per_step, "ms_per_step": round(tot / nruns, 5)}
+        if k in step_bytes:
+            per_launch = step_bytes[k] / per_step
+            gbs = per_launch / (avg * 1e-3) / 1e9
+            passes[k].update({"alg_bytes": per_launch, "GBps": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)})
+    return passes
+
+
+def _plans(batch):
+    return [f"{r['pass']} N={r['N']} {r['kernel']} {r['radices']} T={r['T']} {r['choice']}"
+            + (f" tuned_ms={r['tuned_ms']} default_ms={r['default_ms']}" if "tuned_ms" in r else "")
+            + (" cached" if r.get("cached") else "") for r in batch.plan_report()]
+
+
+def _make(kind, *args, tune=False, mfma=0):
+    """A batch created with the autotuner and/or the MFMA radix-16 stages (1: every pass, 2: the x
+    passes) switched on."""
+    from scarl_amd import _capi as A
+    from scarl_amd import generator as Gm
+    Gm.set_tuning(A.TUNE_AUTOTUNE, int(tune))
+    Gm.set_tuning(A.TUNE_MFMA_DFT16, int(mfma))
+    try:
+        return getattr(Gm.Batch, kind)(*args)
+    finally:
+        Gm.set_tuning(A.TUNE_AUTOTUNE, 0)
+        Gm.set_tuning(A.TUNE_MFMA_DFT16, 0)
+
+
+def _mfma_flops(shapes, x_only=False):
+    """bf16 MFMA flops of one generation with every radix-16 first stage on the matrix cores (or
+    those of the x passes only): a pass of length N over P pencils issues P N / 256 groups of 16
+    columns, each 12 products of 16 x 16 x 32 (2 output halves x 6 split terms), 16384 flop apiece
+    (P N x 768 flop); the 256-point x pass of a 512-wide field runs both of its stages there."""
+    tot = 0
+    for n1, n2, n3 in shapes:
+        passes = [(n1 // 2, n2 * n3, True)]
+        if not x_only:
+            passes += [(n3, (n1 // 2 + 1) * n2, False), (n2, (n1 // 2 + 1) * n3, False)]
+        for N, pens, xp in passes:
+            if N >= 32 and N % 16 == 0:
+                tot += pens * N * 768 * (2 if xp and N == 256 else 1)
+    return tot
+
+
+def config5(args):
+    """256 realizations, edges from {64, ..., 192} by seed, half Gaussian L = 12 and half
+    exponential L = 20, standard-normal marginal, FP32: Stockham, MFMA and autotuned plans."""
+    from scarl_amd import _capi as A
+    from scarl_amd import generator as Gm
+    edges = np.random.default_rng(5).choice([64, 96, 128, 160, 192], size=256)
+    specs = [Gm.FieldSpec(n=(int(e),) * 3, model=1 if i < 128 else 3, corrL=(12.0,) * 3 if i < 128 else (20.0,) * 3,
+                          seed=55, stream=i, margi=1, avg=0.0, std_dev=1.0) for i, e in enumerate(edges)]
+    shapes = [s.n for s in specs]
+    pts = int(sum(int(e) ** 3 for e in edges))
+    sb = _batch_bytes(shapes, 4)
+    out = {"workload": "256 realizations, cube edges drawn by seed 5 from {64,96,128,160,192}, half Gaussian "
+                       "(L=12) half exponential (L=20), standard-normal marginal, FP32, one hipGraph",
+           "points": pts, "alg_bytes_per_point": round(sum(sb.values()) / pts, 3)}
+    flops = _mfma_flops(shapes)
+    for name, tune, mfma in (("stockham", False, 0), ("mfma_bf16x3", False, 1), ("autotuned", True, 0)):
+        log(f"bench: config 5 ensemble, {name}")
+        b = _make("ensemble", specs, A.PREC_FP32, tune=tune, mfma=mfma)
+        try:
+            dt = _steps(b, args.steps, args.warmup)
+            passes = _pass_table(b, 3, sb)
+            r = {"value": round(pts / dt / 1e9, 3), "unit": "Gpoints/s", "ms_per_step": round(dt * 1e3, 4),
+                 "step_frac_of_8TBps": round(sum(sb.values()) / dt / 1e9 / HBM_PEAK_GBS, 4), "passes": passes}
+            if mfma:
+                ms = sum(passes[k]["ms_per_step"] for k in ("h_zpass", "h_ypass", "h_xpass") if k in passes)
+                r["mfma_flop_per_step"] = flops
+                r["mfma_TFLOPs_over_pass_time"] = round(flops / (ms * 1e-3) / 1e12, 2)
+                r["mfma_frac_of_bf16_dense_peak"] = round(flops / (ms * 1e-3) / 2.5e15, 4)
+            if tune:
+                r["plans"] = _plans(b)
+            if name == "stockham":
+                # per-realization moments (fused) and per-axis covariance at lags 0-32 (own pass)
+                Gm.timing(True)
+                Gm.timing_reset()
+                try:
+                    b.lag_covariance(33)
+                    rep = Gm.timing_report()
+                finally:
+                    Gm.timing(False)
+                lag_ms = rep["lag"][1]
+                r["lag_cov_33_ms"] = round(lag_ms, 4)
+                r["with_lag_cov_Gpts"] = round(pts / (dt + lag_ms * 1e-3) / 1e9, 3)
+            out[name] = r
+        finally:
+            b.close()
+    return out
+
+
+def config4(args):
+    """1024 x 1024 x 256 blended from 8 x 8 x 2 subdomains of seeded extents, overlap 16,
+    exponential L = 24, lognormal mean 1 CoV 0.5, FP32, one hipGraph (default and autotuned)."""
+    from scarl_amd import _capi as A
+    from scarl_amd import generator as Gm
+    gdim = (1024, 1024, 256)
+    starts, exts = Gm.seeded_tiling(gdim, (8, 8, 2), 16, (128, 144, 160, 176, 192), 2024)
+    spec = Gm.FieldSpec(n=(2, 2, 2), model=3, corrL=(24.0,) * 3, seed=31, margi=2, avg=1.0, std_dev=0.5)
+    shapes = [(exts[0][i], exts[1][j], exts[2][k]) for k in range(2) for j in range(8) for i in range(8)]
+    gen_pts = sum(a * b * c for a, b, c in shapes)
+    gpts = gdim[0] * gdim[1] * gdim[2]
+    sb = _batch_bytes(shapes, 4)
+    out = {"workload": "global 1024x1024x256 from 8x8x2 subdomains (extents 128-192 by seed 2024, overlap 16, "
+                       "cosine PoU), exponential L=24, lognormal mean 1 CoV 0.5, FP32, one hipGraph",
+           "global_points": gpts, "generated_points": gen_pts}
+    for name, tune in (("default", False), ("autotuned", True)):
+        log(f"bench: config 4 blend, {name}")
+        b = _make("blend", gdim, starts, exts, spec, A.PREC_FP32, tune=tune)
+        try:
+            dt = _steps(b, args.steps, args.warmup)
+            passes = _pass_table(b, 3, sb)
+            out[name] = {"value": round(gpts / dt / 1e9, 3), "unit": "Gpoints/s (global points)",
+                         "generated_Gpts": round(gen_pts / dt / 1e9, 3), "ms_per_step": round(dt * 1e3, 4),
+                         "step_frac_of_8TBps": round(sum(sb.values()) / dt / 1e9 / HBM_PEAK_GBS, 4),
+                         "passes": passes}
+            if tune:
+                out[name]["plans"] = _plans(b)
+        finally:
+            b.close()
+    return out
+
+
+def config1(args):
+    """2D 256 x 256, Gaussian C = exp(-(r/8)^2), sigma 1, standard-normal marginal, FP64, one field
+    from Philox seed 1: throughput (launch-bound at this size) and parity with the oracle."""
+    from scarl_amd import _capi as A
+    from scarl_amd import generator as Gm
+    from oracle import oracle as O
+    spec = Gm.FieldSpec(n=(256, 256, 1), model=A.CM_GAUSSIAN, corrL=(8.0, 8.0, 1.0), seed=1, margi=A.FOM_GAUSSIAN,
+                        avg=0.0, std_dev=1.0)
+    b = Gm.Batch.ensemble([spec], A.PREC_FP64)
+    try:
+        dt = _steps(b, max(args.steps, 20), args.warmup)
+        f = b.output(0)
+    finally:
+        b.close()
+    ref = O.gen_mode_h((256, 256, 1), (1, 1, 1), A.CM_GAUSSIAN, (8.0, 8.0, 1.0), seed=1, stream=0, margi=1, avg=0.0,
+                       std=1.0)
+    return {"workload": "2D 256x256 Gaussian L=8, FP64, one field per graph replay", "value": round(256 ** 2 / dt / 1e9, 4),
+            "unit": "Gpoints/s", "ms_per_field": round(dt * 1e3, 4),
+            "rel_l2_vs_fp64_oracle": float(np.linalg.norm(f - ref) / np.linalg.norm(ref)),
+            "sample_mean": float(f.mean()), "sample_var": float(f.var())}
+
+
+def cpu_baseline_redft01(threads):
+    """Config 2 on the host: phases, the oracle's REDFT01 restatement of gen_std_gauss_FFT
+    (SRC_F/fftw_ScaRL.f90:293-381, OpenMP) and the lognormal marginal (main_ScaRL.f90:1355-1395) --
+    the reference's Generation + Multivar_trans window (main_ScaRL.f90:527-583); one warm-up, then the
+    median of three.  Beside it, scipy.fft.dctn(type=3) (pocketfft, `threads` workers) on the same
+    octant amplitude, with numpy's exp for the marginal."""
+    from oracle import oracle as O
+    Np, xr, cl = (256, 256, 256), (255.0, 255.0, 255.0), (16.0, 16.0, 8.0)
+    m = tuple(v // 2 for v in Np)
+    rng = np.random.default_rng(7)
+
+    def oracle_run():
+        phi = np.asfortranarray(rng.random(m))
+        return O.marginal(O.gen_std_gauss_fft(Np, xr, cl, 1, phi, nthreads=threads), 2, 1.0, 0.3)
+
+    def med(fn, reps=3):
+        fn()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts))
+
+    t_or = med(oracle_run)
+    lib = None
+    try:
+        import scipy.fft as sf
+        mu, sd = O.marginal_params(2, 1.0, 0.3)
+
+        def scipy_run():
+            phi = np.asfortranarray(rng.random(m))
+            a = O.mode_r_amplitude(Np, m, xr, cl, 1, phi)
+            y = sf.dctn(a, type=3, workers=threads)
+            with np.errstate(over="ignore"):  # the reference feeds the un-normalised field to exp
+                return np.exp(y * sd + mu)
+
+        t_sp = med(scipy_run)
+        lib = {"library": "scipy.fft.dctn type 3 (pocketfft) + numpy exp", "workers": threads,
+               "value": round(256 ** 3 / t_sp / 1e9, 5), "unit": "Gpoints/s", "s_per_field": round(t_sp, 4)}
+    except Exception as exc:  # the comparator never costs the bench line
+        lib = {"error": str(exc)}
+    return {"value": round(256 ** 3 / t_or / 1e9, 5), "unit": "Gpoints/s", "cores": threads, "kind": "port",
+            "sample": f"config 2: one 256^3 gen_std_gauss_FFT (REDFT01 FP64 on the octant, phases drawn on the host) "
+                      f"+ lognormal marginal, oracle C restatement with OpenMP, median of 3 after a warm-up: "
+                      f"{t_or:.3f} s per field",
+            "library_comparator": lib}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -230,9 +472,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--batch", type=int, default=4)
     ap.add_argument("--edge", type=int, default=512)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the FP64 oracle check")
     ap.add_argument("--no-deck", action="store_true", help="skip the deck-program figure")
     ap.add_argument("--no-secondary", action="store_true", help="skip the other rows' kernel timings")
+    ap.add_argument("--no-configs", action="store_true", help="skip the config 1, 4 and 5 lines")
+    ap.add_argument("--no-tune", action="store_true", help="headline with the planner's fixed rules")
     ap.add_argument("--probe-only", action="store_true", help="run the copy probe and the headline only")
     ap.add_argument("--traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="write profiles/traffic.json from two rocprofv3 --pmc counter CSVs and exit")
@@ -260,58 +504,21 @@ def main():
     B = args.batch
     specs = [Gm.FieldSpec(n=n, model=A.CM_VON_KARMAN, corrL=L, nu=0.3, seed=2024, stream=i, margi=A.FOM_LOGNORMAL,
                           avg=1.0, std_dev=0.3) for i in range(B)]
-
-    # ---- headline: batched FP32, hipGraph ------------------------------------------------
-    log(f"bench: batched fp32 {B} x {e}^3")
-    batch = Gm.Batch.ensemble(specs, A.PREC_FP32)
-    for _ in range(max(args.warmup, 1)):
-        batch.run(use_graph=True)
-    batch.sync()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        batch.run(use_graph=True)
-    batch.sync()
-    dt = (time.perf_counter() - t0) / args.steps
     pts = B * e ** 3
+    pb = pass_bytes(n, 4)
+    sb = {k: B * v for k, v in pb.items()}
+
+    # ---- headline: batched FP32, hipGraph, autotuned plans --------------------------------
+    tune = not args.no_tune
+    log(f"bench: batched fp32 {B} x {e}^3 ({'autotuned' if tune else 'default'} plans)")
+    batch = _make("ensemble", specs, A.PREC_FP32, tune=tune)
+    plans = _plans(batch)
+    dt = _steps(batch, args.steps, args.warmup)
     value = pts / dt / 1e9
     log(f"bench: {dt * 1e3:.3f} ms/step, {value:.1f} Gpts/s")
-
-    # per-kernel durations with HIP events on the library stream (eager launches)
-    # (with timing on every launch runs alone on the library stream; a pass may take several
-    # launches per step, one per field group)
-    Gm.timing(True)
-    Gm.timing_reset()
     nruns = max(3, min(args.steps, 10))
-    for _ in range(nruns):
-        batch.run(use_graph=False)
-    rep = Gm.timing_report()
-    Gm.timing(False)
-    pb = pass_bytes(n, 4)
-    passes = {}
-    for k, (cnt, tot) in rep.items():
-        avg = tot / cnt
-        per_step = cnt / nruns
-        byts = pb.get(k)
-        passes[k] = {"avg_ms": avg, "launches": cnt, "launches_per_step": per_step, "ms_per_step": tot / nruns}
-        if byts is not None:
-            per_launch = B * byts / per_step
-            gbs = per_launch / (avg * 1e-3) / 1e9
-            passes[k].update({"alg_bytes": per_launch, "GBps": gbs, "frac": gbs / HBM_PEAK_GBS})
-    dom = max((k for k in passes if k in pb), key=lambda k: passes[k]["ms_per_step"])
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", "traffic.json")
-    if os.path.exists(tfile):
-        t = json.load(open(tfile)).get(dom)
-        traffic = None if t is None else t["bytes"]
-    roofline = {"bound": "hbm", "kernel": dom, "achieved": passes[dom]["GBps"], "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": passes[dom]["frac"], "traffic": traffic,
-                "bytes_per_launch": passes[dom]["alg_bytes"], "avg_ms": passes[dom]["avg_ms"],
-                "frac_of_copy_probe": passes[dom]["GBps"] / copy_gbs}
-    step_bytes = sum(pb.values()) * B
-    step_frac = step_bytes / dt / 1e9 / HBM_PEAK_GBS
+    passes = _pass_table(batch, nruns, sb)
     log(f"bench: passes {json.dumps(passes)}")
-
-    # keep field 0 for the error checks
     f32_field0 = batch.output(0)
     moments = batch.moments()
     batch.close()
@@ -319,55 +526,86 @@ def main():
         print(json.dumps({"copy_probe_GBps": copy_gbs, "value": value}), flush=True)
         return
 
+    traffic_all = {}
+    tfile = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tfile):
+        traffic_all = json.load(open(tfile))
+    roof_passes = {}
+    for k in pb:
+        if k in passes:
+            t = traffic_all.get(k)
+            roof_passes[k] = {"bound": "hbm", "achieved": passes[k]["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                              "frac": passes[k]["frac"], "traffic": None if t is None else t["bytes"],
+                              "bytes_per_launch": passes[k]["alg_bytes"], "avg_ms": passes[k]["avg_ms"],
+                              "ms_per_step": passes[k]["ms_per_step"]}
+    dom = max(roof_passes, key=lambda k: roof_passes[k]["ms_per_step"])
+    worst = min(roof_passes, key=lambda k: roof_passes[k]["frac"])
+    roofline = dict(roof_passes[dom], kernel=dom, frac_of_copy_probe=round(roof_passes[dom]["achieved"] / copy_gbs, 4))
+    step_bytes = sum(sb.values())
+    step_frac = step_bytes / dt / 1e9 / HBM_PEAK_GBS
+
+    # ---- the same batch with the planner's fixed rules ------------------------------------
+    default_plans = None
+    if tune:
+        log("bench: batched fp32, default plans")
+        db = _make("ensemble", specs, A.PREC_FP32)
+        try:
+            dtd = _steps(db, args.steps, args.warmup)
+            default_plans = {"value": round(pts / dtd / 1e9, 3), "unit": "Gpoints/s", "ms_per_step": round(dtd * 1e3, 4),
+                             "plans": _plans(db)}
+        finally:
+            db.close()
+
     # ---- the same batch with the x pass as split-precision bf16 MFMA radix-16 stages ----------
-    log("bench: batched fp32, MFMA radix-16 x pass")
-    Gm.set_tuning(A.TUNE_MFMA_DFT16, 1)
-    try:
-        mb = Gm.Batch.ensemble(specs, A.PREC_FP32)
-    finally:
-        Gm.set_tuning(A.TUNE_MFMA_DFT16, 0)
-    for _ in range(max(args.warmup, 1)):
-        mb.run(use_graph=True)
-    mb.sync()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        mb.run(use_graph=True)
-    mb.sync()
-    dtm = (time.perf_counter() - t0) / args.steps
-    Gm.timing(True)
-    Gm.timing_reset()
-    for _ in range(nruns):
-        mb.run(use_graph=False)
-    mrep = Gm.timing_report()
-    Gm.timing(False)
-    mx = mrep["h_xpass"][1] / mrep["h_xpass"][0]
-    m0 = mb.output(0)
-    mb.close()
-    mfma = {"value": round(pts / dtm / 1e9, 3), "unit": "Gpoints/s", "ms_per_step": round(dtm * 1e3, 4),
-            "h_xpass_avg_ms": mx, "h_xpass_GBps": pb["h_xpass"] * B / (mrep["h_xpass"][0] / nruns) / (mx * 1e-3) / 1e9,
-            # per pencil: 2 stages x 2 output tiles (Re, Im) x 6 split products of a 16x16x32 bf16 MFMA
-            "mfma_TFLOPs": (e * e) * 2 * 2 * 6 * (16 * 16 * 32 * 2) / (mx * 1e-3) / 1e12,
-            "mfma_frac_of_bf16_dense_peak": (e * e) * 2 * 2 * 6 * (16 * 16 * 32 * 2) / (mx * 1e-3) / 2.5e15,
-            "rel_l2_vs_fp32_stockham": float(np.linalg.norm(m0.astype(np.float64) - f32_field0) /
-                                             np.linalg.norm(f32_field0.astype(np.float64))),
-            "scheme": "x pass radix-16 stages as [Re -Im; Im Re] bf16 MFMA products, three-term split, "
-                      "six products, FP32 accumulation"}
+    mfma = {"scheme": "radix-16 stages as [Re -Im; Im Re] bf16 MFMA products, three-term split, six products, "
+                      "FP32 accumulation"}
+    m0 = None
+    for mode, name, kind in ((2, "x_pass", "x pass only: 256 = 16 x 16, both stages"),
+                             (1, "all_passes", "every radix-16 first stage (z, y: 512 = 16 x 8 x 4; x: 256, both)")):
+        log(f"bench: batched fp32, MFMA radix-16 stages ({name})")
+        mb = _make("ensemble", specs, A.PREC_FP32, mfma=mode)
+        try:
+            dtm = _steps(mb, args.steps, args.warmup)
+            mpass = _pass_table(mb, nruns, sb)
+            mo = mb.output(0)
+        finally:
+            mb.close()
+        if m0 is None:
+            m0 = mo
+        mflops = _mfma_flops([n] * B, x_only=mode == 2)
+        mks = ("h_xpass",) if mode == 2 else ("h_zpass", "h_ypass", "h_xpass")
+        mms = sum(mpass[k]["ms_per_step"] for k in mks)
+        mfma[name] = {"value": round(pts / dtm / 1e9, 3), "unit": "Gpoints/s", "ms_per_step": round(dtm * 1e3, 4),
+                      "stages": kind, "passes": mpass, "mfma_flop_per_step": mflops,
+                      "mfma_TFLOPs_over_pass_time": round(mflops / (mms * 1e-3) / 1e12, 2),
+                      "mfma_frac_of_bf16_dense_peak": round(mflops / (mms * 1e-3) / 2.5e15, 4),
+                      "rel_l2_vs_fp32_stockham": float(np.linalg.norm(mo.astype(np.float64) - f32_field0) /
+                                                       np.linalg.norm(f32_field0.astype(np.float64)))}
 
     # ---- plain FP64, one field per launch ------------------------------------------------
     log("bench: plain fp64, one field per launch")
-    plain = Gm.Batch.ensemble(specs[:1], A.PREC_FP64)
-    plain.run()
-    plain.sync()
-    ks = max(2, args.steps // 2)
-    t0 = time.perf_counter()
-    for _ in range(ks):
-        plain.run()
-    plain.sync()
-    dtp = (time.perf_counter() - t0) / ks
-    plain_value = e ** 3 / dtp / 1e9
-    plain.close()
+    plain_rows = {}
+    for name, tn in (("default", False), ("autotuned", True)):
+        plain = _make("ensemble", specs[:1], A.PREC_FP64, tune=tn)
+        try:
+            plain.run()
+            plain.sync()
+            ks = max(2, args.steps // 2)
+            t0 = time.perf_counter()
+            for _ in range(ks):
+                plain.run()
+            plain.sync()
+            dtp = (time.perf_counter() - t0) / ks
+            pp = _pass_table(plain, 2, pass_bytes(n, 8))
+            plain_rows[name] = {"value": round(e ** 3 / dtp / 1e9, 3), "unit": "Gpoints/s",
+                                "ms_per_field": round(dtp * 1e3, 4), "passes": pp}
+            if tn:
+                plain_rows[name]["plans"] = _plans(plain)
+        finally:
+            plain.close()
+    plain_value = max(r["value"] for r in plain_rows.values())
 
-    # ---- reference-parity REDFT01 drop-in, 256^3 FP64, inputs resident --------------------
+    # ---- config 2: reference-parity REDFT01 drop-in, 256^3 FP64, inputs resident ------------
     log("bench: REDFT01 drop-in 256^3 fp64")
     hip = C.CDLL("libamdhip64.so")
     dptr = C.c_void_p()
@@ -378,17 +616,29 @@ def main():
     xr = np.full(3, 255.0)
     cl = np.array([16.0, 16.0, 8.0])
     A.call("scarl_hip_set_marginal", 2, 1.0, 0.3)
+    ks = max(2, args.steps // 2)
 
-    def redft():
-        A.check(lib.scarl_hip_gen_std_gauss_fft_dev(dptr, A.ptr(Np), A.ptr(xr), A.ptr(cl), 1, 7, 0, None, 1))
+    def redft_rate(name):
+        fn = getattr(lib, name)
 
-    redft()
-    A.check(lib.scarl_hip_batch_sync(None))
-    t0 = time.perf_counter()
-    for _ in range(ks):
-        redft()
-    A.check(lib.scarl_hip_batch_sync(None))
-    dtr = (time.perf_counter() - t0) / ks
+        def call():
+            A.check(fn(dptr, A.ptr(Np), A.ptr(xr), A.ptr(cl), 1, 7, 0, None, 1))
+
+        call()
+        A.check(lib.scarl_hip_batch_sync(None))
+        t0 = time.perf_counter()
+        for _ in range(ks):
+            call()
+        A.check(lib.scarl_hip_batch_sync(None))
+        return (time.perf_counter() - t0) / ks
+
+    dtr = redft_rate("scarl_hip_gen_std_gauss_fft_dev")
+    dtn = redft_rate("scarl_hip_gen_std_gauss_fft_dev_noalloc")
+    Gm.timing(True)
+    Gm.timing_reset()
+    A.check(lib.scarl_hip_gen_std_gauss_fft_dev(dptr, A.ptr(Np), A.ptr(xr), A.ptr(cl), 1, 7, 0, None, 1))
+    rrep = Gm.timing_report()
+    Gm.timing(False)
     hip.hipFree(dptr)
     # the same call with the host output array of the Fortran interface: PCIe-inclusive rate
     from scarl_amd import fftw_ScaRL
@@ -398,24 +648,33 @@ def main():
     for _ in range(3):
         fftw_ScaRL.gen_std_gauss_FFT(hout, (256, 256, 256), tuple(xr), tuple(cl), 1, 7, 0)
     dth = (time.perf_counter() - t0) / 3
+    rbytes = {"r_xpass": 8 * 256 ** 3 * (1 / 8 + 1 / 4), "r_ypass": 8 * 256 ** 3 * (1 / 4 + 1 / 2),
+              "r_zpass": 8 * 256 ** 3 * (1 / 2 + 1)}
+    rpass = {k: {"ms": round(v[1] / v[0], 5), "GBps": round(rbytes[k] / (v[1] / v[0] * 1e-3) / 1e9, 1),
+                 "frac": round(rbytes[k] / (v[1] / v[0] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+             for k, v in rrep.items() if k in rbytes}
 
-    # ---- CPU baseline (oracle, FP64) and FP32 error on shared noise ----------------------
+    # ---- configs 1, 4, 5 --------------------------------------------------------------------
+    c1 = c4 = c5 = None
+    if not args.no_configs:
+        c1 = config1(args)
+        c4 = config4(args)
+        c5 = config5(args)
+
+    # ---- CPU baseline (config 2 on the host) and FP32 error on shared noise -----------------
     cpu = None
     err = None
     if not args.no_cpu:
         from oracle import oracle as O
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        log(f"bench: CPU oracle, one {e}^3 field, {threads} threads")
-        t0 = time.perf_counter()
+        log(f"bench: CPU baseline, config 2 256^3 REDFT01, {threads} threads")
+        cpu = cpu_baseline_redft01(threads)
+        log(f"bench: FP64 oracle of headline field 0, {threads} threads")
         ref = O.gen_mode_h(n, (1, 1, 1), A.CM_VON_KARMAN, L, 0.3, seed=2024, stream=0, margi=A.FOM_LOGNORMAL,
                            avg=1.0, std=0.3, nthreads=threads)
-        tc = time.perf_counter() - t0
-        cpu = {"value": e ** 3 / tc / 1e9, "unit": "Gpoints/s", "cores": threads, "kind": "port",
-               "sample": f"one {e}^3 lognormal von Karman field (same spec and Philox noise as field 0), "
-                         f"FP64 oracle C port, {tc:.2f} s"}
         err = {"rel_l2": float(np.linalg.norm(f32_field0 - ref) / np.linalg.norm(ref)),
                "max_abs": float(np.abs(f32_field0.astype(np.float64) - ref).max()),
-               "mfma_rel_l2": float(np.linalg.norm(m0 - ref) / np.linalg.norm(ref))}
+                      "mfma_x_pass_rel_l2": float(np.linalg.norm(m0 - ref) / np.linalg.norm(ref))}
 
     # ---- the other rows' kernels on one 256^3 FP64 field ------------------------------------
     secondary = None
@@ -444,23 +703,32 @@ def main():
         "dtype": "fp32",
         "data": "synthetic: device Philox4x32-10 Hermitian white noise",
         "config": {"workload": f"batched Hermitian C2R, {B} x {e}^3 lognormal von Karman (L=32 cells, nu=0.3, "
-                               f"mean 1, CoV 0.3), one hipGraph per step",
+                               f"mean 1, CoV 0.3), one hipGraph per step, "
+                               f"{'autotuned' if tune else 'default'} plans",
                    "fields_per_step": B, "edge": e, "precision": "fp32 butterflies, fp64-rounded-once twiddles"},
         "roofline": roofline,
+        "roofline_passes": roof_passes,
+        "roofline_least_efficient": dict(roof_passes[worst], kernel=worst),
         "step_roofline_frac": round(step_frac, 4),
         "copy_probe": {"GBps": round(copy_gbs, 1), "bytes": COPY_PROBE_BYTES, "avg_ms": round(copy_ms, 4),
                        "step_frac_of_copy": round(step_bytes / dt / 1e9 / copy_gbs, 4)},
         "alg_bytes_per_point": sum(pb.values()) / e ** 3,
         "passes": passes,
-        "plain_fp64": {"value": round(plain_value, 3), "unit": "Gpoints/s", "ms_per_field": round(dtp * 1e3, 4),
-                       "workload": f"one {e}^3 field per launch, FP64"},
+        "plans": plans,
+        "default_plans": default_plans,
+        "plain_fp64": dict(plain_rows[max(plain_rows, key=lambda k: plain_rows[k]["value"])],
+                           workload=f"one {e}^3 field per launch, FP64", variants=plain_rows),
         "batched_vs_plain": round(value / plain_value, 3),
         "dropin_redft01": {"value": round(256 ** 3 / dtr / 1e9, 3), "unit": "Gpoints/s",
-                           "ms_per_field": round(dtr * 1e3, 4),
-                           "workload": "gen_std_gauss_FFT 256^3 FP64 + lognormal, device phi, resident"},
+                           "ms_per_field": round(dtr * 1e3, 4), "passes": rpass,
+                           "noalloc_value": round(256 ** 3 / dtn / 1e9, 3),
+                           "workload": "config 2: gen_std_gauss_FFT 256^3 FP64 + lognormal, device phi, resident"},
         "dropin_redft01_host_array": {"value": round(256 ** 3 / dth / 1e9, 3), "unit": "Gpoints/s",
                                       "ms_per_field": round(dth * 1e3, 4),
                                       "workload": "the same call into a host array (PCIe copy included)"},
+        "config1_2d": c1,
+        "config4_blend": c4,
+        "config5_ensemble": c5,
         "fp32_vs_fp64": err,
         "mfma_variant": mfma,
         "moments_field0": {"mean": float(moments[0, 3]), "std": float(moments[0, 4])},

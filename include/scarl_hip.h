@@ -48,7 +48,7 @@ int scarl_hip_get_stream(void** stream);
  * SCARL_TUNE_MFMA_DFT16: 1 runs every FP32 radix-16 first stage (lengths that are multiples of 16,
  * 32 to 4096; both stages of the 256-point x pass of 512-wide fields) as split-precision bf16 matrix
  * products on the MFMA units (three bf16 terms per operand, six products, FP32 accumulation);
- * 0 (default) keeps the FP32 Stockham butterflies.
+ * 2 does so in the x passes only; 0 (default) keeps the FP32 Stockham butterflies.
  * SCARL_TUNE_AUTOTUNE: 1 makes batch creation time every plan's candidates (compile-time kernel,
  * runtime radix splits with 16 or 32 elements per thread in either order, pencils per tile, LDS
  * skew, MFMA first stage) on the batch's own passes and keep the fastest; picks are cached per

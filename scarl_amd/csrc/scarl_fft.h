@@ -28,8 +28,10 @@ constexpr double kPiRef = 3.1415926535898;
 // register budget; radices above EC are never planned for the instantiation).
 struct DefaultStageIn {};
 
-template <typename T, int LAYOUT, bool SIN, bool SOUT, int EC, class Prep, class GLoad, class GStore,
-          class GIn = DefaultStageIn>
+// NS: the plan is a length with a prime factor above 13 (Bluestein, or the direct O(N^2) DFT); those
+// two paths live in their own instantiation so that they cost the Stockham kernels nothing.
+template <typename T, int LAYOUT, bool SIN, bool SOUT, int EC, bool NS = false, class Prep, class GLoad,
+          class GStore, class GIn = DefaultStageIn>
 __device__ __forceinline__ void pencil_fft(const PlanDev& pl, const cplx<T>* __restrict__ pool, cplx<T>* lds,
                                            int npen, Prep&& prep, GLoad&& gld, GStore&& gst, GIn&& gin = GIn()) {
     const int tid = threadIdx.x;
@@ -53,8 +55,54 @@ __device__ __forceinline__ void pencil_fft(const PlanDev& pl, const cplx<T>* __r
     auto ld_l = [&](int n) -> cplx<T> { return lds[addr(n)]; };
     auto st_l = [&](int n, cplx<T> v) { lds[addr(n)] = v; };
 
-    if (pl.nst == 0) {
-        // direct O(N^2) DFT (lengths with a prime factor above 13, and N = 1)
+    if constexpr (NS) {
+    if (pl.blu_M > 0) {
+        // Bluestein: X_k = c_k sum_j (x_j c_j) conj(c_{k-j}), c_j = exp(+i pi j^2/N), as the circular
+        // convolution of length M >= 2N - 1: with a = (x c, zero padded) and E = DFT_M^-1(conj c)/M,
+        // conv = conj(DFT_M^-1(conj(DFT_M^-1(a) E))); both transforms are the backward Stockham stages
+        // of M, in place in the tile.
+        const int M = pl.blu_M, S = pl.nst;
+        const cplx<T>* ch = pool + pl.chirp_off;
+        const cplx<T>* E = pool + pl.kern_off;
+        auto put = [&](int n, cplx<T> v) { st_l(n, cmul(v, ch[n])); };
+        if (active) {
+            if constexpr (SIN && !std::is_same<typename std::decay<GIn>::type, DefaultStageIn>::value) {
+                gin(ctx, b, tpp, put);
+            } else {
+#pragma unroll 1
+                for (int n = b; n < N; n += tpp) put(n, ld_g(n));
+            }
+#pragma unroll 1
+            for (int n = N + b; n < M; n += tpp) st_l(n, cplx<T>{T(0), T(0)});
+        }
+        __syncthreads();
+        for (int pass = 0; pass < 2; ++pass) {
+            for (int s = 0; s < S; ++s) {
+                stage_dispatch<EC, T>(pl.R[s], M, pl.Ns[s], pool + pl.tw_off[s], b, tpp, active, true, ld_l, st_l);
+                __syncthreads();
+            }
+            if (pass == 0) {
+                if (active) {
+#pragma unroll 1
+                    for (int n = b; n < M; n += tpp) {
+                        const cplx<T> v = cmul(ld_l(n), E[n]);
+                        st_l(n, cplx<T>{v.x, -v.y});
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        if (active) {
+#pragma unroll 1
+            for (int k = b; k < N; k += tpp) {
+                const cplx<T> v = ld_l(k);
+                st_g(k, cmul(cplx<T>{v.x, -v.y}, ch[k]));
+            }
+        }
+        return;
+    }
+    {
+        // direct O(N^2) DFT (short lengths with a prime factor above 13, and N = 1)
 #pragma unroll 1
         for (int n = b; n < N; n += tpp)
             if (active) st_l(n, ld_g(n));
@@ -75,6 +123,7 @@ __device__ __forceinline__ void pencil_fft(const PlanDev& pl, const cplx<T>* __r
         }
         return;
     }
+    } else {
     if constexpr (SIN) {
         if (active) {
             if constexpr (std::is_same<typename std::decay<GIn>::type, DefaultStageIn>::value) {
@@ -112,6 +161,7 @@ __device__ __forceinline__ void pencil_fft(const PlanDev& pl, const cplx<T>* __r
 #pragma unroll 1
         for (int n = b; n < N; n += tpp)
             if (active) st_g(n, ld_l(n));
+    }
     }
 }
 
@@ -335,7 +385,18 @@ struct GenericFft {
     template <typename T, int LAYOUT, bool SIN, bool SOUT, class... A>
     __device__ __forceinline__ static void run(const PlanDev& pl, const cplx<T>* __restrict__ pool, cplx<T>* lds,
                                                int npen, A&&... a) {
-        pencil_fft<T, LAYOUT, SIN, SOUT, EC>(pl, pool, lds, npen, static_cast<A&&>(a)...);
+        pencil_fft<T, LAYOUT, SIN, SOUT, EC, false>(pl, pool, lds, npen, static_cast<A&&>(a)...);
+    }
+};
+// lengths with a prime factor above 13 (kSpecNonSmooth): Bluestein or the direct DFT
+template <int EC>
+struct NonSmoothFft {
+    static constexpr bool kDirectGen = false;
+    static constexpr bool kXPair = false;
+    template <typename T, int LAYOUT, bool SIN, bool SOUT, class... A>
+    __device__ __forceinline__ static void run(const PlanDev& pl, const cplx<T>* __restrict__ pool, cplx<T>* lds,
+                                               int npen, A&&... a) {
+        pencil_fft<T, LAYOUT, SIN, SOUT, EC, true>(pl, pool, lds, npen, static_cast<A&&>(a)...);
     }
 };
 template <int N, int R0, int R1, int TPP>

@@ -27,6 +27,9 @@ struct PlanDev {
     int ecap;              // complex elements a thread holds per stage (generic kernel instantiation)
     int spec;              // compile-time planned kernel id (SCARL_FIXED_SPECS), 0 = generic
     int skew;              // LAYOUT_BF runtime plans: 1 = one LDS element of skew per 16 (ldp includes it)
+    int blu_M;             // > 0: Bluestein, the stages are those of the inner length blu_M >= 2N - 1
+    int chirp_off;         // Bluestein chirp exp(+i pi j^2/N), j < N
+    int kern_off;          // Bluestein kernel: the length-blu_M backward DFT of exp(-i pi m^2/N) (wrapped), / blu_M
 };
 
 enum SourceMode { SRC_WORKSPACE = 0, SRC_DEVICE_NOISE = 1, SRC_HOST_NOISE = 2 };
@@ -125,6 +128,9 @@ constexpr int kSpecMfma256 = 64;
 // elements (four 16-column groups per wave).
 constexpr int kSpecMfmaGeneric = 65;
 constexpr int kMfmaTileElems = 4096;
+// Runtime plans of lengths with a prime factor above 13: Bluestein (blu_M > 0) or the direct
+// O(N^2) DFT (nst = 0), run by their own kernel instantiation (NonSmoothFft in scarl_fft.h).
+constexpr int kSpecNonSmooth = 66;
 
 // Blocks per field of the tile-partial reduction (k_mom_field); the per-field buffer holds
 // kMomChunks partials per field.

@@ -386,6 +386,11 @@ struct ModeHDispatch {
             if constexpr (PREC == 4) strided<GenericMfmaFft<E1>>(axis, gen, fields, tiles, tile0, ntiles, plans, p, lds, s);
             return hipGetLastError();
         }
+        if (spec == kSpecNonSmooth) {
+            if (ecap == E2) strided<NonSmoothFft<E2>>(axis, gen, fields, tiles, tile0, ntiles, plans, p, lds, s);
+            else strided<NonSmoothFft<E1>>(axis, gen, fields, tiles, tile0, ntiles, plans, p, lds, s);
+            return hipGetLastError();
+        }
         switch (spec) {
 #define SCARL_CASE(ID, P, L, N, R0, R1, TPP)                                                             \
     case ID:                                                                                             \
@@ -408,8 +413,19 @@ struct ModeHDispatch {
         if (lds < 4 * kBlock * sizeof(double)) lds = 4 * kBlock * sizeof(double);
         const cplx<T>* p = static_cast<const cplx<T>*>(pool);
         if (full) {
-            if (ecap == E2) xpass<GenericFft<E2>, true>(blend, fields, tiles, tile0, ntiles, plans, p, partials, lds, s);
-            else xpass<GenericFft<E1>, true>(blend, fields, tiles, tile0, ntiles, plans, p, partials, lds, s);
+            if (spec == kSpecNonSmooth) {
+                if (ecap == E2) xpass<NonSmoothFft<E2>, true>(blend, fields, tiles, tile0, ntiles, plans, p, partials, lds, s);
+                else xpass<NonSmoothFft<E1>, true>(blend, fields, tiles, tile0, ntiles, plans, p, partials, lds, s);
+            } else if (ecap == E2) {
+                xpass<GenericFft<E2>, true>(blend, fields, tiles, tile0, ntiles, plans, p, partials, lds, s);
+            } else {
+                xpass<GenericFft<E1>, true>(blend, fields, tiles, tile0, ntiles, plans, p, partials, lds, s);
+            }
+            return hipGetLastError();
+        }
+        if (spec == kSpecNonSmooth) {
+            if (ecap == E2) xpass<NonSmoothFft<E2>>(blend, fields, tiles, tile0, ntiles, plans, p, partials, lds, s);
+            else xpass<NonSmoothFft<E1>>(blend, fields, tiles, tile0, ntiles, plans, p, partials, lds, s);
             return hipGetLastError();
         }
         if (spec == kSpecMfma256) {

@@ -131,7 +131,7 @@ struct Context {
     int margi = 0;
     double m_avg = 0.0, m_std = 1.0;
     int fp32_ecap = 16;  // register budget (complex elements per thread) of new FP32 plans
-    bool mfma_dft16 = false;  // FP32 radix-16 first stages on the matrix cores (new batches)
+    int mfma_dft16 = 0;       // FP32 radix-16 first stages on the matrix cores: 1 all passes, 2 x passes
     bool autotune = false;    // time every plan's candidates at batch creation
     void* stage[2] = {nullptr, nullptr};         // pinned staging buffers for caller arrays
     hipEvent_t stage_ev[2] = {nullptr, nullptr};  // last DMA that used each staging buffer
@@ -316,7 +316,7 @@ static void cis_ld(long long num, long long den, long double& c, long double& s)
 // One way to plan a transform length (the autotuner's search space).  kind 0 reproduces the
 // fixed rules: the compile-time planned kernel of SCARL_FIXED_SPECS when one matches (or the MFMA
 // kernels when the set asks for them), else the runtime plan with the set's register budget.
-enum PlanVariant { PV_DEFAULT = 0, PV_FIXED = 1, PV_GENERIC = 2, PV_MFMA = 3 };
+enum PlanVariant { PV_DEFAULT = 0, PV_FIXED = 1, PV_GENERIC = 2, PV_MFMA = 3, PV_DIRECT = 4 };
 struct PlanChoice {
     int variant = PV_DEFAULT;
     int spec = 0;   // PV_FIXED: SCARL_FIXED_SPECS id
@@ -332,6 +332,7 @@ static std::string choice_name(const PlanChoice& c) {
     case PV_FIXED: snprintf(b, sizeof b, "fixed:%d", c.spec); break;
     case PV_GENERIC: snprintf(b, sizeof b, "generic:e%d:o%d:t%d:s%d", c.ecap, c.order, c.tdiv, c.skew); break;
     case PV_MFMA: snprintf(b, sizeof b, "mfma:t%d:s%d", c.tdiv, c.skew); break;
+    case PV_DIRECT: snprintf(b, sizeof b, "direct"); break;
     default: snprintf(b, sizeof b, "default"); break;
     }
     return b;
@@ -340,6 +341,10 @@ static std::string choice_name(const PlanChoice& c) {
 static bool parse_choice(const char* txt, PlanChoice& c) {
     c = PlanChoice();
     if (strcmp(txt, "default") == 0) return true;
+    if (strcmp(txt, "direct") == 0) {
+        c.variant = PV_DIRECT;
+        return true;
+    }
     if (sscanf(txt, "fixed:%d", &c.spec) == 1) { c.variant = PV_FIXED; return true; }
     if (sscanf(txt, "generic:e%d:o%d:t%d:s%d", &c.ecap, &c.order, &c.tdiv, &c.skew) == 4) {
         c.variant = PV_GENERIC;
@@ -352,6 +357,37 @@ static bool parse_choice(const char* txt, PlanChoice& c) {
     return false;
 }
 
+// Backward DFT (exp(+2 pi i jk/n)) of a 2^a 3^b 5^c length in long double, recursive mixed radix.
+static void dft_ld(std::vector<long double>& re, std::vector<long double>& im) {
+    const int n = (int)re.size();
+    if (n == 1) return;
+    const int p = n % 2 == 0 ? 2 : n % 3 == 0 ? 3 : 5;
+    const int m = n / p;
+    std::vector<std::vector<long double>> sr(p, std::vector<long double>(m)), si(p, std::vector<long double>(m));
+    for (int r = 0; r < p; ++r)
+        for (int j = 0; j < m; ++j) {
+            sr[r][j] = re[(size_t)p * j + r];
+            si[r][j] = im[(size_t)p * j + r];
+        }
+    for (int r = 0; r < p; ++r) dft_ld(sr[r], si[r]);
+    for (int k = 0; k < n; ++k) {
+        long double ar = 0.0L, ai = 0.0L;
+        for (int r = 0; r < p; ++r) {
+            long double c, sn;
+            cis_ld((long long)r * k, n, c, sn);
+            const long double xr = sr[r][k % m], xi = si[r][k % m];
+            ar += xr * c - xi * sn;
+            ai += xr * sn + xi * c;
+        }
+        re[k] = ar;
+        im[k] = ai;
+    }
+}
+
+// Inner length of a Bluestein plan: the 2^a 3^b 5^c length M >= 2N - 1 (up to twice that) with the
+// least M x (Stockham stages at register budget ec).
+static int bluestein_length(int N, int ec);
+
 struct PlanSet {
     std::vector<PlanDev> plans;
     std::vector<long double> pool;  // interleaved re/im, long double; rounded once at upload
@@ -362,7 +398,7 @@ struct PlanSet {
     PlanDev* d_plans = nullptr;
     void* d_pool = nullptr;
     bool fp64 = false;
-    bool mfma = false;   // plan the FP32 x/y/z passes with MFMA radix-16 stages where the length allows
+    int mfma = 0;        // MFMA radix-16 stages where the length allows: 1 every FP32 pass, 2 the x passes
     int ecap = kEcap64;  // register budget of the default runtime plans (matches a kernel instantiation)
     size_t max_lds = 0;
 
@@ -467,12 +503,13 @@ struct PlanSet {
         const bool mfma_ok = !fp64 && kind != 2 && N >= 32 && N % 16 == 0 && N <= kMfmaTileElems;
         int variant = ch.variant;
         if (variant == PV_DEFAULT) {
-            if (mfma && mfma_ok && !(fspec && layout == LAYOUT_BF && kind == 1 && N == 256 && sr0 == 16 &&
+            if (mfma && (mfma == 1 || layout == LAYOUT_BF) && mfma_ok && !(fspec && layout == LAYOUT_BF && kind == 1 && N == 256 && sr0 == 16 &&
                                      sr1 == 16 && stpp == 16))
                 variant = PV_MFMA;
             else variant = fspec ? PV_FIXED : PV_GENERIC;
         }
         int tdiv = ch.variant == PV_DEFAULT ? 1 : ch.tdiv;
+        if (variant == PV_DIRECT && factor(N, ecap, rad)) return undo(SCARL_E_UNSUPPORTED);
         if (variant == PV_FIXED) {
             const int spec = ch.variant == PV_FIXED ? ch.spec : fspec;
             if (spec != fspec || !spec) return SCARL_E_UNSUPPORTED;
@@ -513,9 +550,48 @@ struct PlanSet {
             pl.spec = 0;
             pl.ecap = ec;
             stages(pl, rad, ec, false);
+        } else if (N >= 32 && ch.variant != PV_DIRECT && ch.variant != PV_FIXED && ch.variant != PV_MFMA) {
+            // a prime factor above 13: Bluestein over the inner length M
+            const int ec = ch.variant == PV_GENERIC ? ch.ecap : ecap;
+            if (fp64 && ec != kEcap64) return undo(SCARL_E_UNSUPPORTED);
+            const int M = bluestein_length(N, ec);
+            if (M <= 0 || !factor(M, ec, rad)) return -fail(SCARL_E_UNSUPPORTED, "no Bluestein length for %d", N);
+            if (ch.variant == PV_GENERIC) {
+                if (ch.order) std::reverse(rad.begin(), rad.end());
+                pl.skew = ch.skew;
+            }
+            pl.spec = kSpecNonSmooth;
+            pl.ecap = ec;
+            pl.N = M;  // the stages and threads per pencil are those of M
+            stages(pl, rad, ec, false);
+            pl.N = N;
+            pl.blu_M = M;
+            pl.chirp_off = (int)(pool.size() / 2);
+            for (long long j = 0; j < N; ++j) {
+                long double c, sn;
+                cis_ld((j * j) % (2LL * N), 2LL * N, c, sn);  // exp(+i pi j^2/N)
+                pool_push(c, sn);
+            }
+            // E = DFT_M^-1(d)/M, d_m = exp(-i pi m^2/N) for |m| < N (wrapped mod M), 0 elsewhere
+            std::vector<long double> dr(M, 0.0L), di(M, 0.0L);
+            for (long long m = 0; m < N; ++m) {
+                long double c, sn;
+                cis_ld((m * m) % (2LL * N), 2LL * N, c, sn);
+                dr[m] = c;
+                di[m] = -sn;
+                if (m) {
+                    dr[M - m] = c;
+                    di[M - m] = -sn;
+                }
+            }
+            dft_ld(dr, di);
+            pl.kern_off = (int)(pool.size() / 2);
+            for (int m = 0; m < M; ++m) pool_push(dr[m] / M, di[m] / M);
         } else {
-            if (ch.variant != PV_DEFAULT && ch.variant != PV_GENERIC) return undo(SCARL_E_UNSUPPORTED);
-            pl.nst = 0;  // direct DFT (N == 1 or a prime factor above 13)
+            if (ch.variant != PV_DEFAULT && ch.variant != PV_GENERIC && ch.variant != PV_DIRECT)
+                return undo(SCARL_E_UNSUPPORTED);
+            pl.nst = 0;  // direct DFT (N == 1, or a short length with a prime factor above 13)
+            pl.spec = kSpecNonSmooth;
             if (N > 4096) return -fail(SCARL_E_UNSUPPORTED, "length %d has a prime factor above 13 and exceeds 4096", N);
             pl.direct_off = (int)(pool.size() / 2);
             for (int t = 0; t < N; ++t) {
@@ -530,16 +606,18 @@ struct PlanSet {
             return -fail(SCARL_E_UNSUPPORTED, "transform length %d too large for one tile", N);
         }
         pl.T = kBlock / pl.tpp;
-        pl.ldp = N + (pl.skew ? N / 16 : 0) + 1;
+        const int NL = pl.blu_M > 0 ? pl.blu_M : N;  // elements of one pencil in the LDS tile
+        pl.ldp = NL + (pl.skew ? NL / 16 : 0) + 1;
         // keep the LDS tile within 64 KiB so two tiles fit on a CU.  Every plan gets an LDS
         // tile, because passes that stage their input or output use it even for one stage.
         // (compile-time planned kernels fix T = 256 / TPP; the spec table respects the bound)
-        const size_t per = (size_t)(layout == LAYOUT_PF ? N : pl.ldp) * csz;
+        const size_t per = (size_t)(layout == LAYOUT_PF ? NL : pl.ldp) * csz;
         const int Tmax = (int)std::max<size_t>(1, (64 * 1024) / per);
-        if (!pl.spec) pl.T = std::min(pl.T, Tmax);
+        const bool runtime_tile = pl.spec == 0 || pl.spec == kSpecMfmaGeneric || pl.spec == kSpecNonSmooth;
+        if (runtime_tile) pl.T = std::min(pl.T, Tmax);
         if (pl.spec == kSpecMfmaGeneric) pl.T = std::max(1, std::min(pl.T, kMfmaTileElems / N));
-        if (pl.spec == 0 || pl.spec == kSpecMfmaGeneric) pl.T = std::max(1, pl.T / tdiv);
-        pl.lds_elems = (int)(pl.T * (layout == LAYOUT_PF ? N : pl.ldp));
+        if (runtime_tile) pl.T = std::max(1, pl.T / tdiv);
+        pl.lds_elems = (int)(pl.T * (layout == LAYOUT_PF ? NL : pl.ldp));
         if (kind == 1) {
             pl.pre_off = (int)(pool.size() / 2);
             for (int k = 0; k < N; ++k) {
@@ -610,6 +688,25 @@ struct PlanSet {
         return 0;
     }
 };
+
+static int bluestein_length(int N, int ec) {
+    const long long lo = 2LL * N - 1;
+    int best = -1;
+    long long best_cost = 0;
+    std::vector<int> rad;
+    for (long long M = lo; M <= 2 * lo && M <= 16384; ++M) {
+        long long r = M;
+        for (int p : {2, 3, 5})
+            while (r % p == 0) r /= p;
+        if (r != 1 || !PlanSet::factor((int)M, ec, rad)) continue;
+        const long long cost = M * (long long)rad.size();
+        if (best < 0 || cost < best_cost) {
+            best = (int)M;
+            best_cost = cost;
+        }
+    }
+    return best;
+}
 
 static int check_lds(size_t bytes) {
     if (bytes > 160 * 1024) return fail(SCARL_E_UNSUPPORTED, "tile needs %zu bytes of LDS", bytes);
@@ -1131,6 +1228,11 @@ static std::vector<PlanChoice> plan_candidates(const PlanSet& ps, int N, int lay
                     g.skew = sk;
                     c.push_back(g);
                 }
+    {
+        PlanChoice d;
+        d.variant = PV_DIRECT;  // applies only to lengths with a prime factor above 13
+        c.push_back(d);
+    }
     if (!ps.fp64)
         for (int tdiv : {1, 2})
             for (int sk : skews) {
@@ -1148,7 +1250,7 @@ static std::string plan_signature(const PlanDev& p) {
     std::string s = std::to_string(p.spec) + "/" + std::to_string(p.ecap) + "/" + std::to_string(p.T) + "/" +
                     std::to_string(p.tpp) + "/" + std::to_string(p.ldp) + "/" + std::to_string(p.skew) + "/";
     for (int i = 0; i < p.nst; ++i) s += std::to_string(p.R[i]) + ",";
-    return s;
+    return s + "/" + std::to_string(p.blu_M);
 }
 
 // Average time of the batch's launches of one plan (every group), eager on the library stream.
@@ -1186,8 +1288,10 @@ static std::string plan_line(const PlanSet& ps, int i) {
     std::string rad;
     for (int s = 0; s < p.nst; ++s) rad += (s ? "x" : "") + std::to_string(p.R[s]);
     if (p.nst == 0) rad = "direct";
+    if (p.blu_M > 0) rad = "bluestein" + std::to_string(p.blu_M) + ":" + rad;
     const char* kern = p.spec == kSpecMfmaGeneric ? "mfma-generic" : p.spec == kSpecMfma256 ? "mfma-256"
-                                                                    : p.spec ? "fixed" : "generic";
+                       : p.spec == kSpecNonSmooth ? (p.blu_M > 0 ? "bluestein" : "direct")
+                       : p.spec ? "fixed" : "generic";
     snprintf(buf, sizeof buf, "pass=%s N=%d kernel=%s spec=%d radices=%s ecap=%d T=%d tpp=%d ldp=%d skew=%d choice=%s",
              role >= 0 && role < 3 ? roles[role] : "?", N, kern, p.spec, rad.c_str(), p.ecap, p.T, p.tpp, p.ldp,
              p.skew, choice_name(ps.choices[i]).c_str());
@@ -1397,47 +1501,73 @@ static int mode_r_run(double* d_out, const int32_t n[3], const int32_t m[3], con
 
 // The same passes with every device buffer (plan table, twiddle pool, tmp1, tmp2) carved out of
 // the library's static scratch array (scarl_noalloc.hip, noalloc_scratch()): no device memory is
-// allocated.  The plan table and the pool are copied into the scratch in stream order (after any
-// earlier work on the library stream has read them), then the host waits for that copy, so the
-// host tables may die on return; the passes themselves run asynchronously, as in the _dev call.
+// allocated.  When the shape differs from the last call's on this device, the plan table and the
+// pool are copied into the scratch in stream order and the host waits for that copy; the passes
+// themselves run asynchronously, as in the _dev call.
+// Per device: the shape whose plan table and twiddles the scratch holds (a repeated shape skips
+// the planning, the copy and its wait).
+struct NoAllocTables {
+    bool valid = false;
+    int n[3] = {0, 0, 0}, m[3] = {0, 0, 0};
+    std::vector<PlanDev> plans;
+    int plan[3] = {0, 0, 0};
+    size_t lds[3] = {0, 0, 0};
+    size_t o_pool = 0, o_t1 = 0, o_t2 = 0;
+};
+static NoAllocTables g_noalloc_tables[64];
+
 static int mode_r_run_noalloc(double* d_out, const int32_t n[3], const int32_t m[3], const double xRange[3],
                               const double corrL[3], int32_t corrMod, uint64_t seed, const double* d_phi,
                               int32_t flags) {
     size_t cap = 0;
     char* base = static_cast<char*>(noalloc_scratch(g_ctx.device, &cap));
     if (!base) return fail(SCARL_E_HIP, "%s", noalloc_error(g_ctx.device));
+    if (g_ctx.device < 0 || g_ctx.device >= 64) return fail(SCARL_E_INVALID, "device index out of range");
     FieldR f;
     SCARL_TRY(mode_r_desc(f, d_out, n, m, xRange, corrL, corrMod, seed, d_phi, flags));
-    PlanSet ps;  // host side only: upload() is never called, so it owns no device memory
-    ps.fp64 = true;
-    size_t lds[3];
-    for (int a = 0; a < 3; ++a) {
-        f.plan[a] = ps.get(n[a], a == 0 ? LAYOUT_BF : LAYOUT_PF, 2);
-        if (f.plan[a] < 0) return -f.plan[a];
+    NoAllocTables& tb = g_noalloc_tables[g_ctx.device];
+    const bool same = tb.valid && memcmp(tb.n, n, sizeof tb.n) == 0 && memcmp(tb.m, m, sizeof tb.m) == 0;
+    if (!same) {
+        tb.valid = false;
+        PlanSet ps;  // host side only: upload() is never called, so it owns no device memory
+        ps.fp64 = true;
+        for (int a = 0; a < 3; ++a) {
+            tb.plan[a] = ps.get(n[a], a == 0 ? LAYOUT_BF : LAYOUT_PF, 2);
+            if (tb.plan[a] < 0) return -tb.plan[a];
+        }
+        for (int a = 0; a < 3; ++a) {
+            tb.lds[a] = (size_t)ps.plans[tb.plan[a]].lds_elems * 16;
+            SCARL_TRY(check_lds(tb.lds[a]));
+        }
+        // layout: [plan table | twiddle pool (FP64 complex) | tmp1 | tmp2], each 256-byte aligned
+        auto up = [](size_t v) { return (v + 255) & ~size_t(255); };
+        std::vector<double> pool(std::max<size_t>(2, ps.pool.size()), 0.0);
+        for (size_t i = 0; i < ps.pool.size(); ++i) pool[i] = (double)ps.pool[i];
+        const size_t plan_bytes = ps.plans.size() * sizeof(PlanDev), pool_bytes = pool.size() * sizeof(double);
+        int64_t t1, t2;
+        mode_r_tmp_elems(n, m, &t1, &t2);
+        tb.o_pool = up(plan_bytes);
+        tb.o_t1 = tb.o_pool + up(pool_bytes);
+        tb.o_t2 = tb.o_t1 + up((size_t)t1 * 8);
+        const size_t need = tb.o_t2 + up((size_t)t2 * 8);
+        if (need > cap)
+            return fail(SCARL_E_INVALID,
+                        "Np = (%d, %d, %d) needs %zu bytes of scratch; the allocation-free entry point holds %zu "
+                        "(workspaces up to 96 MiB, e.g. 256^3 points)", n[0], n[1], n[2], need, cap);
+        // in stream order, after any earlier call's kernels have read the old tables; the host
+        // waits so that the host tables may die on return
+        HIP_TRY(hipMemcpyAsync(base, ps.plans.data(), plan_bytes, hipMemcpyHostToDevice, g_ctx.stream));
+        HIP_TRY(hipMemcpyAsync(base + tb.o_pool, pool.data(), pool_bytes, hipMemcpyHostToDevice, g_ctx.stream));
+        HIP_TRY(hipStreamSynchronize(g_ctx.stream));
+        tb.plans = ps.plans;
+        memcpy(tb.n, n, sizeof tb.n);
+        memcpy(tb.m, m, sizeof tb.m);
+        tb.valid = true;
     }
-    for (int a = 0; a < 3; ++a) {
-        lds[a] = (size_t)ps.plans[f.plan[a]].lds_elems * 16;
-        SCARL_TRY(check_lds(lds[a]));
-    }
-    // layout: [plan table | twiddle pool (FP64 complex) | tmp1 | tmp2], each 256-byte aligned
-    auto up = [](size_t v) { return (v + 255) & ~size_t(255); };
-    std::vector<double> pool(std::max<size_t>(2, ps.pool.size()), 0.0);
-    for (size_t i = 0; i < ps.pool.size(); ++i) pool[i] = (double)ps.pool[i];
-    const size_t plan_bytes = ps.plans.size() * sizeof(PlanDev), pool_bytes = pool.size() * sizeof(double);
-    int64_t t1, t2;
-    mode_r_tmp_elems(n, m, &t1, &t2);
-    const size_t o_pool = up(plan_bytes), o_t1 = o_pool + up(pool_bytes), o_t2 = o_t1 + up((size_t)t1 * 8);
-    const size_t need = o_t2 + up((size_t)t2 * 8);
-    if (need > cap)
-        return fail(SCARL_E_INVALID,
-                    "Np = (%d, %d, %d) needs %zu bytes of scratch; the allocation-free entry point holds %zu "
-                    "(workspaces up to 96 MiB, e.g. 256^3 points)", n[0], n[1], n[2], need, cap);
-    HIP_TRY(hipMemcpyAsync(base, ps.plans.data(), plan_bytes, hipMemcpyHostToDevice, g_ctx.stream));
-    HIP_TRY(hipMemcpyAsync(base + o_pool, pool.data(), pool_bytes, hipMemcpyHostToDevice, g_ctx.stream));
-    HIP_TRY(hipStreamSynchronize(g_ctx.stream));
-    f.tmp1 = reinterpret_cast<double*>(base + o_t1);
-    f.tmp2 = reinterpret_cast<double*>(base + o_t2);
-    return mode_r_passes(f, ps.plans.data(), reinterpret_cast<const PlanDev*>(base), base + o_pool, lds, false);
+    for (int a = 0; a < 3; ++a) f.plan[a] = tb.plan[a];
+    f.tmp1 = reinterpret_cast<double*>(base + tb.o_t1);
+    f.tmp2 = reinterpret_cast<double*>(base + tb.o_t2);
+    return mode_r_passes(f, tb.plans.data(), reinterpret_cast<const PlanDev*>(base), base + tb.o_pool, tb.lds, false);
 }
 
 }  // namespace scarl
@@ -1531,8 +1661,8 @@ int scarl_hip_set_tuning(int32_t key, int32_t value) {
         return 0;
     }
     if (key == SCARL_TUNE_MFMA_DFT16) {
-        if (value != 0 && value != 1) return fail(SCARL_E_INVALID, "SCARL_TUNE_MFMA_DFT16 takes 0 or 1");
-        g_ctx.mfma_dft16 = value != 0;
+        if (value < 0 || value > 2) return fail(SCARL_E_INVALID, "SCARL_TUNE_MFMA_DFT16 takes 0, 1 or 2");
+        g_ctx.mfma_dft16 = value;
         return 0;
     }
     if (key == SCARL_TUNE_AUTOTUNE) {

@@ -62,3 +62,17 @@ def test_traffic_needs_the_copy_probe(tmp_path):
     _counter_csv(tmp_path / "w.csv", "WRITE_SIZE", [("void scarl::k_h_xpass<float, F>", 1.0)])
     with pytest.raises(SystemExit):
         b.traffic_from_counters(str(tmp_path / "f.csv"), str(tmp_path / "w.csv"), str(tmp_path / "t.json"))
+
+
+def test_mfma_flop_count():
+    """bf16 MFMA flops per generation: P pencils of a length N (multiple of 16, at least 32) cost
+    P N / 256 groups of 16 columns x 12 products x 16384 flop; the 256-point x pass of a 512-wide
+    field runs both of its radix-16 stages on the matrix cores."""
+    b = _bench()
+    per_group = 12 * 16 * 16 * 32 * 2
+    z = 257 * 512 * (512 // 16 // 16) * per_group
+    x = 512 * 512 * (256 // 16 // 16) * per_group * 2
+    assert b._mfma_flops([(512, 512, 512)]) == 2 * z + x
+    # 24 along x: the half-length 12 has no radix-16 stage; y = 16 is below the 32-point minimum
+    assert b._mfma_flops([(24, 16, 64)]) == 13 * 16 * 64 * 768
+    assert b._mfma_flops([(512, 512, 512)], x_only=True) == x

@@ -781,6 +781,53 @@ def test_autotune_cache_file(hip, tmp_path):
     assert [c + "/cached" for c in first] == second
 
 
+# Lengths with a prime factor above 13 (the reference's geometry gives 41, 233, 241, 461): Bluestein
+# over an inner 2^a 3^b 5^c length M >= 2N - 1 for N >= 32, two backward Stockham FFTs of M
+BLUESTEIN_SHAPES = [(241, 233, 41), (482, 40, 24), (461, 48, 32), (233, 241, 1), (64, 461, 34)]
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("case", range(len(BLUESTEIN_SHAPES)))
+def test_bluestein_lengths_mode_h(hip, case, prec):
+    from scarl_amd import generator
+    n = BLUESTEIN_SHAPES[case]
+    _say(f"bluestein mode H n={n} fp{prec}")
+    L = (12.0, 9.0, 5.0)
+    spec = generator.FieldSpec(n=n, model=2, corrL=L, nu=0.6, seed=9, stream=case, margi=2, avg=1.0, std_dev=0.4)
+    with generator.Batch.ensemble([spec], prec) as b:
+        b.run()
+        out = b.output(0)
+        rep = b.plan_report()
+    assert any(r["radices"].startswith("bluestein") for r in rep), rep
+    ref = O.gen_mode_h(n, (1, 1, 1), 2, L, 0.6, seed=9, stream=case, margi=2, avg=1.0, std=0.4)
+    assert rel_l2(out, ref) <= (FP64_TOL if prec == 64 else FP32_TOL)
+
+
+@pytest.mark.parametrize("Np", [(241, 40, 233), (41, 461, 36)])
+def test_bluestein_redft01(hip, Np):
+    """gen_std_gauss_FFT (FP64 REDFT01) on prime extents against the oracle, 1e-12."""
+    from scarl_amd import fftw_ScaRL
+    xr = tuple(float(v - 1) for v in Np)
+    cl = (8.0, 6.0, 5.0)
+    phi = np.asfortranarray(np.random.default_rng(41).random(tuple(v // 2 for v in Np)))
+    out = np.empty(Np, np.float64, order="F")
+    fftw_ScaRL.gen_std_gauss_FFT(out, Np, xr, cl, 1, 3, 0, phi=phi)
+    assert rel_l2(out, O.gen_std_gauss_fft(Np, xr, cl, 1, phi)) <= FP64_TOL
+
+
+def test_bluestein_deck_size(hip):
+    """The reference's MAX_Test deck at SIZE 115 gives 241^3 subdomains: one blended FP64 batch of
+    2 x 2 x 2 such subdomains against the oracle's gather-sum on a reduced z extent (241 x 241 x 48)."""
+    from scarl_amd import generator
+    starts, exts = [[0, 220], [0, 220], [0]], [[241, 241], [241, 241], [48]]
+    gdim = (461, 461, 48)
+    spec = generator.FieldSpec(n=(2, 2, 2), model=1, corrL=(20.0, 20.0, 20.0), seed=5, margi=2, avg=1.0, std_dev=1.0)
+    with generator.Batch.blend(gdim, starts, exts, spec, 64) as b:
+        b.run()
+        G = b.output(-1)
+    assert rel_l2(G, _blend_reference(gdim, starts, exts, spec)) <= FP64_TOL
+
+
 def test_deck_geometry_blend(hip):
     """test/3_TEST sample 1 (x in [0, 10], corrL 1, overlap 1, 5 points per corrL, Gaussian,
     lognormal 1/1, seed 1) on a 2x2x2 topology: the eight ranks' subdomains (23^3, overlap 5,
