@@ -53,8 +53,12 @@ int scarl_hip_get_stream(void** stream);
  * runtime radix splits with 16 or 32 elements per thread in either order, pencils per tile, LDS
  * skew, MFMA first stage) on the batch's own passes and keep the fastest; picks are cached per
  * process, and in the file named by the SCARL_TUNE_CACHE environment variable when it is set
- * (the reference's counterpart is FFTW's planner, SRC_F/fftw_ScaRL.f90:378). */
-enum { SCARL_TUNE_FP32_ELEMS = 1, SCARL_TUNE_MFMA_DFT16 = 2, SCARL_TUNE_AUTOTUNE = 3 };
+ * (the reference's counterpart is FFTW's planner, SRC_F/fftw_ScaRL.f90:378).
+ * SCARL_TUNE_LAG_FFT: 1 (default) computes the periodic lag sums of the corrL estimator by FFT
+ * (|FFT|^2 and back, FP64, as SRC_F/Stat_ScaRL/statistics_RF.f90:353-448 does) wherever the length
+ * has a Stockham plan and there are at least 8 lags; 0 uses the direct products from LDS.  The
+ * bounded lags of the per-axis covariance always use the direct products. */
+enum { SCARL_TUNE_FP32_ELEMS = 1, SCARL_TUNE_MFMA_DFT16 = 2, SCARL_TUNE_AUTOTUNE = 3, SCARL_TUNE_LAG_FFT = 4 };
 int scarl_hip_set_tuning(int32_t key, int32_t value);
 
 /* ---- fftw_ScaRL drop-in ------------------------------------------------------------------ */

@@ -159,6 +159,8 @@ struct LagJob {
     double scale;
     double* partial;        // [block of this job][nlag]
     int64_t nblocks;
+    int fft_plan;           // >= 0: lag sums by FFT (k_lag_fft) over the FP64 plan of length P; -1: direct
+    int P;                  // FFT length: n (circular) or >= n + nlag - 1 (zero padded, bounded pairs)
 };
 struct LagBlock {
     int job;

@@ -275,6 +275,88 @@ __global__ __launch_bounds__(kBlock) void k_lag_partial(const LagJob* __restrict
     }
 }
 
+// The same partial sums by FFT, as the reference estimates them (SRC_F/Stat_ScaRL/statistics_RF.f90:
+// 353-448): a block holds 2 T real pencils (T = the plan's pencils per tile) as T complex pencils
+// z = u_a + i u_b of length P in LDS (zero padded past n when the pairs are bounded), transforms them,
+// forms W_k = (|X_a,k|^2, |X_b,k|^2) from Z_k and Z_{P-k} (each pair handled by one thread), and
+// transforms W: Re and Im of the result at l are P times the lag-l sums of u_a and u_b (circular over
+// P, which equals the bounded sum when P >= n + nlag - 1).  FP64 throughout.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_lag_fft(const LagJob* __restrict__ jobs, const LagBlock* __restrict__ blocks,
+                                                     const PlanDev* __restrict__ plans,
+                                                     const cplx<double>* __restrict__ pool) {
+    HIP_DYNAMIC_SHARED(char, smem)
+    cplx<double>* L = reinterpret_cast<cplx<double>*>(smem);
+    double* Ld = reinterpret_cast<double*>(smem);
+    const LagBlock bl = blocks[blockIdx.x];
+    const LagJob& J = jobs[bl.job];
+    const PlanDev& pl = plans[J.fft_plan];
+    const int ax = J.axis, n = J.n[ax], nlag = J.nlag, P = J.P;
+    const int Tc = pl.T, TR = 2 * Tc, tpp = pl.tpp;
+    const int oa = ax == 0 ? 1 : 0, ob = ax == 2 ? 1 : 2;  // the other axes, oa fastest
+    const int64_t st[3] = {1, J.ld1, J.ld2};
+    const int64_t npen = (int64_t)J.n[oa] * J.n[ob];
+    const int np = (int)min((int64_t)TR, npen - bl.pencil0);
+    const T* __restrict__ x = static_cast<const T*>(J.x);
+    const double c = J.centre ? *J.centre : J.cval;
+    for (int idx = threadIdx.x; idx < n * TR; idx += kBlock) {
+        int rp, m;
+        if (ax == 0) {  // rows are contiguous: consecutive threads walk a row
+            rp = idx / n;
+            m = idx - rp * n;
+        } else {        // neighbouring pencils are neighbours in x
+            rp = idx % TR;
+            m = idx / TR;
+        }
+        double v = 0.0;
+        if (rp < np) {
+            const int64_t Pn = bl.pencil0 + rp;
+            const int64_t a = Pn % J.n[oa], b = Pn / J.n[oa];
+            v = ((double)x[a * st[oa] + b * st[ob] + (int64_t)m * st[ax]] - c) * J.scale;
+            if (J.window) v *= stat_taper(m, n);
+        }
+        Ld[2 * (m * Tc + (rp >> 1)) + (rp & 1)] = v;
+    }
+    for (int idx = threadIdx.x; idx < (P - n) * Tc; idx += kBlock) L[n * Tc + idx] = cplx<double>{0.0, 0.0};
+    __syncthreads();
+    const int p = threadIdx.x % Tc, b = threadIdx.x / Tc;
+    const bool active = b < tpp;
+    auto ld_l = [&](int k) -> cplx<double> { return L[k * Tc + p]; };
+    auto st_l = [&](int k, cplx<double> v) { L[k * Tc + p] = v; };
+    for (int pass = 0; pass < 2; ++pass) {
+        for (int s = 0; s < pl.nst; ++s) {
+            stage_dispatch<kEcap64, double>(pl.R[s], P, pl.Ns[s], pool + pl.tw_off[s], b, tpp, active, true, ld_l,
+                                            st_l);
+            __syncthreads();
+        }
+        if (pass == 0) {
+            if (active) {
+                for (int k = b; 2 * k <= P; k += tpp) {
+                    const int kk = k ? P - k : 0;
+                    const cplx<double> za = ld_l(k), zb = ld_l(kk);
+                    const double sr = za.x + zb.x, si = za.y - zb.y;  // Z_k + conj Z_{-k} = 2 X_a
+                    const double dr = za.x - zb.x, di = za.y + zb.y;  // Z_k - conj Z_{-k} = 2i X_b
+                    const cplx<double> w = {0.25 * (sr * sr + si * si), 0.25 * (dr * dr + di * di)};
+                    st_l(k, w);
+                    st_l(kk, w);
+                }
+            }
+            __syncthreads();
+        }
+    }
+    const int64_t bj = bl.pencil0 / TR;
+    const double inv = 1.0 / (double)P;
+    for (int l = threadIdx.x; l < nlag; l += kBlock) {
+        double s = 0.0;
+        if (l < n)
+            for (int q = 0; q < Tc; ++q) {
+                const cplx<double> v = L[l * Tc + q];
+                s += v.x + v.y;
+            }
+        J.partial[bj * nlag + l] = s * inv;
+    }
+}
+
 // out[j][l] = (Neumaier sum over the job's blocks) / (pairs per pencil * pencils); undefined lags
 // (l >= n) give NaN.  Grid (nlag_max, njobs).
 __global__ __launch_bounds__(kBlock) void k_lag_finish(const LagJob* __restrict__ jobs, int nlag_max,
@@ -321,11 +403,23 @@ __global__ void k_normalize(double* __restrict__ x, int64_t n, double a, double 
         x[i] = (x[i] - a) / b;
 }
 
+// blocks[0, nblocks) run the direct products (LDS lds), fblocks[0, nfblocks) the FFT path (LDS flds,
+// plans and pool of FP64 complex twiddles); then the compensated combination of every job.
 hipError_t launch_lag(bool fp64, const LagJob* jobs, int njobs, const LagBlock* blocks, int64_t nblocks,
-                      size_t lds, int nlag_max, double* out, hipStream_t s) {
+                      size_t lds, const LagBlock* fblocks, int64_t nfblocks, size_t flds, const PlanDev* plans,
+                      const void* pool, int nlag_max, double* out, hipStream_t s) {
     if (nblocks > 0) {
         if (fp64) hipLaunchKernelGGL(k_lag_partial<double>, dim3((unsigned)nblocks), dim3(kBlock), lds, s, jobs, blocks);
         else hipLaunchKernelGGL(k_lag_partial<float>, dim3((unsigned)nblocks), dim3(kBlock), lds, s, jobs, blocks);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    if (nfblocks > 0) {
+        const cplx<double>* p = static_cast<const cplx<double>*>(pool);
+        if (fp64)
+            hipLaunchKernelGGL(k_lag_fft<double>, dim3((unsigned)nfblocks), dim3(kBlock), flds, s, jobs, fblocks, plans, p);
+        else
+            hipLaunchKernelGGL(k_lag_fft<float>, dim3((unsigned)nfblocks), dim3(kBlock), flds, s, jobs, fblocks, plans, p);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }

@@ -58,6 +58,7 @@ hipError_t launch_marginal(double* x, int64_t n, int margi, double mu, double sd
 hipError_t launch_pou(double* x, int n1, int n2, int n3, int3 lo, int3 hi, hipStream_t s);
 hipError_t launch_sums(const double* rf, int64_t n, int nmc, int nblocks, MomPartial* partial, hipStream_t s);
 hipError_t launch_lag(bool fp64, const LagJob* jobs, int njobs, const LagBlock* blocks, int64_t nblocks, size_t lds,
+                      const LagBlock* fblocks, int64_t nfblocks, size_t flds, const PlanDev* plans, const void* pool,
                       int nlag_max, double* out, hipStream_t s);
 hipError_t launch_box_sums(const double* x, const int n[3], const int box[3], double centre, MomPartial* partial,
                            int nblocks, hipStream_t s);
@@ -133,6 +134,7 @@ struct Context {
     int fp32_ecap = 16;  // register budget (complex elements per thread) of new FP32 plans
     int mfma_dft16 = 0;       // FP32 radix-16 first stages on the matrix cores: 1 all passes, 2 x passes
     bool autotune = false;    // time every plan's candidates at batch creation
+    bool lag_fft = true;      // periodic lag sums by FFT where the length plans (else direct products)
     void* stage[2] = {nullptr, nullptr};         // pinned staging buffers for caller arrays
     hipEvent_t stage_ev[2] = {nullptr, nullptr};  // last DMA that used each staging buffer
 };
@@ -1398,6 +1400,10 @@ struct ModeRCache {
     size_t lds[3] = {0, 0, 0};
 };
 static ModeRCache g_r;
+// FP64 plans of the FFT path of the lag sums (lengths P), kept for the process and uploaded when
+// they grow.
+static std::unique_ptr<PlanSet> g_lag_ps;
+static size_t g_lag_uploaded = 0;
 
 static int mode_r_prepare(const int32_t n[3], const int32_t m[3]) {
     if (g_r.ps && memcmp(g_r.n, n, sizeof g_r.n) == 0 && memcmp(g_r.m, m, sizeof g_r.m) == 0) return 0;
@@ -1597,6 +1603,8 @@ int scarl_hip_finalize(void) {
     g_ctx.pending.clear();
     g_r.ps.reset();
     memset(g_r.n, 0, sizeof g_r.n);
+    g_lag_ps.reset();
+    g_lag_uploaded = 0;
     for (int i = 0; i < 2; ++i) {
         if (g_ctx.stage_ev[i]) {
             hipEventSynchronize(g_ctx.stage_ev[i]);
@@ -1663,6 +1671,11 @@ int scarl_hip_set_tuning(int32_t key, int32_t value) {
     if (key == SCARL_TUNE_MFMA_DFT16) {
         if (value < 0 || value > 2) return fail(SCARL_E_INVALID, "SCARL_TUNE_MFMA_DFT16 takes 0, 1 or 2");
         g_ctx.mfma_dft16 = value;
+        return 0;
+    }
+    if (key == SCARL_TUNE_LAG_FFT) {
+        if (value != 0 && value != 1) return fail(SCARL_E_INVALID, "SCARL_TUNE_LAG_FFT takes 0 or 1");
+        g_ctx.lag_fft = value != 0;
         return 0;
     }
     if (key == SCARL_TUNE_AUTOTUNE) {
@@ -1873,28 +1886,63 @@ int scarl_hip_moments(const double* h_rf, int64_t n, int32_t nmc, double* evnt_a
 }
 
 // ---- lag products: per-axis covariance and the corrL estimator ------------------------------
-// Runs every job in one launch (block table over jobs) and writes [job][nlag_max] to d_out.
-// Pencils per block: the largest power of two (at most 32) whose (n + nlag) T doubles fit 64 KiB.
+// The FFT length and plan of a periodic lag job (the corrL estimator): P = n; -1 when the direct
+// products must be used (the switch is off, fewer than 8 lags, bounded pairs, a length only
+// Bluestein could plan, or P > 4096).  Bounded pairs (the per-axis covariance, a few dozen lags)
+// stay on the direct products: zero padding to P >= n + nlag - 1 made them slower than the products.
+static int lag_fft_plan(const LagJob& J, int* P_out) {
+    const int n = J.n[J.axis];
+    if (!g_ctx.lag_fft || J.nlag < 8 || !J.circular) return -1;
+    if (!g_lag_ps) {
+        g_lag_ps.reset(new PlanSet());
+        g_lag_ps->fp64 = true;
+    }
+    std::vector<int> rad;
+    const int P = n;
+    if (P < 2 || P > 4096 || !PlanSet::factor(P, kEcap64, rad)) return -1;
+    const int id = g_lag_ps->get(P, LAYOUT_PF, 0, 3);
+    if (id < 0 || g_lag_ps->plans[id].spec == kSpecNonSmooth) return -1;
+    *P_out = P;
+    return id;
+}
+
+// Runs every job in one pass per method (block tables over jobs) and writes [job][nlag_max] to d_out.
+// Direct products: pencils per block the largest power of two (at most 32) whose (n + nlag) T doubles
+// fit 64 KiB.  FFT: 2 T real pencils per block, T the plan's complex pencils per tile.
 static int run_lag_jobs(std::vector<LagJob>& jobs, bool fp64, int nlag_max, double* d_out) {
-    size_t lds = 0;
+    size_t lds = 0, flds = 0;
     int64_t tot_partial = 0;
-    std::vector<LagBlock> blocks;
+    std::vector<LagBlock> blocks, fblocks;
     for (size_t j = 0; j < jobs.size(); ++j) {
         LagJob& J = jobs[j];
         const int n = J.n[J.axis];
-        if ((size_t)(n + J.nlag) * 8 > 65536)
-            return fail(SCARL_E_UNSUPPORTED, "a pencil of %d points with %d lags exceeds the 64 KiB tile", n, J.nlag);
-        int T = 32;
-        while (T > 1 && (size_t)(n + J.nlag) * T * 8 > 65536) T >>= 1;
-        J.T = T;
         const int64_t npen = (int64_t)J.n[0] * J.n[1] * J.n[2] / n;
         if (npen > INT32_MAX) return fail(SCARL_E_UNSUPPORTED, "too many pencils (%lld)", (long long)npen);
+        int P = 0;
+        J.fft_plan = lag_fft_plan(J, &P);
+        int T;
+        if (J.fft_plan >= 0) {
+            const PlanDev& pl = g_lag_ps->plans[J.fft_plan];
+            J.P = P;
+            T = 2 * pl.T;
+            flds = std::max(flds, (size_t)pl.T * P * 16);
+        } else {
+            if ((size_t)(n + J.nlag) * 8 > 65536)
+                return fail(SCARL_E_UNSUPPORTED, "a pencil of %d points with %d lags exceeds the 64 KiB tile", n, J.nlag);
+            T = 32;
+            while (T > 1 && (size_t)(n + J.nlag) * T * 8 > 65536) T >>= 1;
+            lds = std::max(lds, (size_t)(n + J.nlag) * T * 8);
+        }
+        J.T = T;
         J.nblocks = (npen + T - 1) / T;
-        for (int64_t p = 0; p < npen; p += T) blocks.push_back({(int)j, (int)p});
+        for (int64_t p = 0; p < npen; p += T) (J.fft_plan >= 0 ? fblocks : blocks).push_back({(int)j, (int)p});
         tot_partial += J.nblocks * J.nlag;
-        lds = std::max(lds, (size_t)(n + J.nlag) * T * 8);
     }
-    void *d_part, *d_jobs, *d_blocks;
+    if (g_lag_ps && g_lag_ps->plans.size() != g_lag_uploaded) {
+        SCARL_TRY(g_lag_ps->upload());
+        g_lag_uploaded = g_lag_ps->plans.size();
+    }
+    void *d_part, *d_jobs, *d_blocks, *d_fblocks;
     SCARL_TRY(scratch_get("lag_partial", (size_t)std::max<int64_t>(1, tot_partial) * 8, &d_part));
     int64_t off = 0;
     for (LagJob& J : jobs) {
@@ -1903,13 +1951,20 @@ static int run_lag_jobs(std::vector<LagJob>& jobs, bool fp64, int nlag_max, doub
     }
     SCARL_TRY(scratch_get("lag_jobs", jobs.size() * sizeof(LagJob), &d_jobs));
     SCARL_TRY(scratch_get("lag_blocks", std::max<size_t>(1, blocks.size()) * sizeof(LagBlock), &d_blocks));
+    SCARL_TRY(scratch_get("lag_fblocks", std::max<size_t>(1, fblocks.size()) * sizeof(LagBlock), &d_fblocks));
     SCARL_TRY(copy_to_device(d_jobs, jobs.data(), jobs.size() * sizeof(LagJob)));
     if (!blocks.empty())
         HIP_TRY(hipMemcpyAsync(d_blocks, blocks.data(), blocks.size() * sizeof(LagBlock), hipMemcpyHostToDevice,
                                g_ctx.stream));
+    if (!fblocks.empty())
+        HIP_TRY(hipMemcpyAsync(d_fblocks, fblocks.data(), fblocks.size() * sizeof(LagBlock), hipMemcpyHostToDevice,
+                               g_ctx.stream));
+    SCARL_TRY(check_lds(flds));
     SCARL_TRY(timed("lag", [&] {
         return launch_lag(fp64, (const LagJob*)d_jobs, (int)jobs.size(), (const LagBlock*)d_blocks,
-                          (int64_t)blocks.size(), lds, nlag_max, d_out, g_ctx.stream);
+                          (int64_t)blocks.size(), lds, (const LagBlock*)d_fblocks, (int64_t)fblocks.size(), flds,
+                          g_lag_ps ? g_lag_ps->d_plans : nullptr, g_lag_ps ? g_lag_ps->d_pool : nullptr, nlag_max,
+                          d_out, g_ctx.stream);
     }));
     HIP_TRY(hipStreamSynchronize(g_ctx.stream));  // the host job and block tables die on return
     return 0;

@@ -489,6 +489,36 @@ def test_rebuild_sk_fft_matches_oracle(hip):
         assert abs(corrl[d] - exp) <= 1e-12 * max(1.0, abs(exp))
 
 
+@pytest.mark.parametrize("lag_fft", [1, 0], ids=["fft", "direct"])
+def test_lag_sums_both_paths(hip, lag_fft):
+    """The periodic lag sums by FFT (|FFT|^2 and back) and by direct products give the oracle's
+    corrL estimator to 1e-12, and the per-axis covariance (bounded lags, direct products on both
+    settings) stays exact.  n = 47 has no Stockham plan, so its periodic lags fall back to the direct
+    products even on the FFT path; 512 periodic lags take a 512-point FFT."""
+    from scarl_amd import _capi as A, generator, statistics
+    rng = np.random.default_rng(23)
+    generator.set_tuning(A.TUNE_LAG_FFT, lag_fft)
+    try:
+        x3 = np.asfortranarray(rng.standard_normal((47, 40, 36)) * 0.7 - 0.1)
+        ref = O.rebuild_sk_corrl(x3, 1.0, -0.1, 0.7)
+        for d in (1, 2, 3):
+            sk = statistics.rebuild_Sk_FFT(x3, x3.shape, d, 3, -0.1, 0.7)
+            np.testing.assert_allclose(sk, ref[d - 1], rtol=0, atol=1e-12 * np.abs(ref[d - 1]).max())
+        x1 = np.asfortranarray(rng.standard_normal((512, 24, 3)))
+        ref1 = O.rebuild_sk_corrl(x1, 1.0, 0.0, 1.0)
+        sk = statistics.rebuild_Sk_FFT(x1, x1.shape, 1, 3, 0.0, 1.0)
+        np.testing.assert_allclose(sk, ref1[0], rtol=0, atol=1e-12 * np.abs(ref1[0]).max())
+        x = np.asfortranarray(rng.standard_normal((47, 33, 192)) + 1.0)
+        mu = float(x.mean())
+        cov = statistics.lag_covariance(x, mu, 33)
+        rc = O.lag_covariance(x, mu, 33)
+        assert np.array_equal(np.isnan(cov), np.isnan(rc))
+        ok = ~np.isnan(rc)
+        np.testing.assert_allclose(cov[ok], rc[ok], rtol=0, atol=1e-12 * np.abs(rc[ok]).max())
+    finally:
+        generator.set_tuning(A.TUNE_LAG_FFT, 1)
+
+
 def test_lag_covariance_matches_oracle(hip):
     from scarl_amd import statistics
     rng = np.random.default_rng(22)
