@@ -235,23 +235,26 @@ __global__ __launch_bounds__(kBlock) void k_lag_partial(const LagJob* __restrict
     const int np = (int)min((int64_t)TT, npen - bl.pencil0);
     const T* __restrict__ x = static_cast<const T*>(J.x);
     const double c = J.centre ? *J.centre : J.cval;
-    for (int idx = threadIdx.x; idx < n * TT; idx += kBlock) {
-        int p, m;
-        if (ax == 0) {  // rows are contiguous: consecutive threads walk a row
-            p = idx / n;
-            m = idx - p * n;
-        } else {        // neighbouring pencils are neighbours in x
-            p = idx % TT;
-            m = idx / TT;
-        }
+    __shared__ int64_t pbase[32];  // element 0 of each pencil of the block
+    if (threadIdx.x < TT) {
+        const int64_t P = bl.pencil0 + threadIdx.x;
+        pbase[threadIdx.x] = (P % J.n[oa]) * st[oa] + (P / J.n[oa]) * st[ob];
+    }
+    __syncthreads();
+    auto put = [&](int p, int m) {
         double v = 0.0;
         if (p < np) {
-            const int64_t P = bl.pencil0 + p;
-            const int64_t a = P % J.n[oa], b = P / J.n[oa];
-            v = ((double)x[a * st[oa] + b * st[ob] + (int64_t)m * st[ax]] - c) * J.scale;
+            v = ((double)x[pbase[p] + (int64_t)m * st[ax]] - c) * J.scale;
             if (J.window) v *= stat_taper(m, n);
         }
         u[m * TT + p] = v;
+    };
+    if (ax == 0) {  // rows are contiguous: consecutive threads walk a row
+        for (int p = 0; p < TT; ++p)
+            for (int m = threadIdx.x; m < n; m += kBlock) put(p, m);
+    } else {        // neighbouring pencils are neighbours in x
+        const int G = kBlock / TT, p = threadIdx.x % TT;
+        for (int m = threadIdx.x / TT; m < n && threadIdx.x < G * TT; m += G) put(p, m);
     }
     __syncthreads();
     double* red = u + (int64_t)n * TT;  // [l][p]
@@ -299,23 +302,28 @@ __global__ __launch_bounds__(kBlock) void k_lag_fft(const LagJob* __restrict__ j
     const int np = (int)min((int64_t)TR, npen - bl.pencil0);
     const T* __restrict__ x = static_cast<const T*>(J.x);
     const double c = J.centre ? *J.centre : J.cval;
-    for (int idx = threadIdx.x; idx < n * TR; idx += kBlock) {
-        int rp, m;
-        if (ax == 0) {  // rows are contiguous: consecutive threads walk a row
-            rp = idx / n;
-            m = idx - rp * n;
-        } else {        // neighbouring pencils are neighbours in x
-            rp = idx % TR;
-            m = idx / TR;
-        }
+    __shared__ int64_t pbase[2 * kBlock];  // element 0 of each real pencil of the block (TR <= 2 kBlock)
+    for (int t = threadIdx.x; t < TR; t += kBlock) {
+        const int64_t Pn = bl.pencil0 + t;
+        pbase[t] = (Pn % J.n[oa]) * st[oa] + (Pn / J.n[oa]) * st[ob];
+    }
+    __syncthreads();
+    auto put = [&](int rp, int m) {
         double v = 0.0;
         if (rp < np) {
-            const int64_t Pn = bl.pencil0 + rp;
-            const int64_t a = Pn % J.n[oa], b = Pn / J.n[oa];
-            v = ((double)x[a * st[oa] + b * st[ob] + (int64_t)m * st[ax]] - c) * J.scale;
+            v = ((double)x[pbase[rp] + (int64_t)m * st[ax]] - c) * J.scale;
             if (J.window) v *= stat_taper(m, n);
         }
         Ld[2 * (m * Tc + (rp >> 1)) + (rp & 1)] = v;
+    };
+    if (ax == 0) {  // rows are contiguous: consecutive threads walk a row
+        for (int rp = 0; rp < TR; ++rp)
+            for (int m = threadIdx.x; m < n; m += kBlock) put(rp, m);
+    } else if (TR <= kBlock) {  // neighbouring pencils are neighbours in x
+        const int G = kBlock / TR, rp = threadIdx.x % TR;
+        for (int m = threadIdx.x / TR; m < n && threadIdx.x < G * TR; m += G) put(rp, m);
+    } else {
+        for (int idx = threadIdx.x; idx < n * TR; idx += kBlock) put(idx % TR, idx / TR);
     }
     for (int idx = threadIdx.x; idx < (P - n) * Tc; idx += kBlock) L[n * Tc + idx] = cplx<double>{0.0, 0.0};
     __syncthreads();
