@@ -302,26 +302,41 @@ __global__ __launch_bounds__(kBlock) void k_h_xpass(const FieldH* __restrict__ f
             }
             gstore(reinterpret_cast<cplx<T>*>(o + 2 * k), cplx<T>{r0, r1});
         } else {
+        // Outside every overlap (mask 0) a point has one owner: no weight, no read, marginal and
+        // moments at once.  Even n1: the two outputs of z go out as one 2-element store when neither
+        // reads what an earlier owner wrote and the row is aligned for it.
         const T wyz = c.wyz;
         const int myz = c.myz;
         const int ne = full ? 1 : 2;
-        for (int e = 0; e < ne; ++e) {
+        T val[2];
+        bool rd[2];
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            if (e >= ne) break;
             const int i = full ? k : 2 * k + e;
-            const T w = wyz * pou_sqrt_weight<T>(i, n1, f.nov_lo[0], f.nov_hi[0]);
-            T val = (e ? z.y : z.x) * sqrt(w);
-            const int msk = myz | (in_overlap(i, n1, f.nov_lo[0], f.nov_hi[0]) ? 1 : 0);
-            const bool first = (f.color & msk) == 0;
-            const bool last = (f.color & msk) == msk;
-            if (!first) val += gload1(o + i);
-            if (last) {
-                val = apply_margi<T>(val, f.margi, mu, sd);
+            const bool ox = in_overlap(i, n1, f.nov_lo[0], f.nov_hi[0]);
+            const int msk = myz | (ox ? 1 : 0);
+            T v = e ? z.y : z.x;
+            if (msk) {
+                const T w = ox ? wyz * pou_sqrt_weight<T>(i, n1, f.nov_lo[0], f.nov_hi[0]) : wyz;
+                v *= sqrt(w);
+            }
+            rd[e] = (f.color & msk) != 0;  // not the first owner: add to what is there
+            if (rd[e]) v += gload1(o + i);
+            if ((f.color & msk) == msk) {  // the last owner
+                v = apply_margi<T>(v, f.margi, mu, sd);
                 if (mom) {
-                    const double d = val;
+                    const double d = v;
                     ms += d;
                     ms2 += d * d;
                 }
             }
-            gstore1(o + i, val);
+            val[e] = v;
+        }
+        if (!full && !rd[0] && !rd[1] && (reinterpret_cast<uintptr_t>(o) % (2 * sizeof(T))) == 0) {
+            gstore(reinterpret_cast<cplx<T>*>(o + 2 * k), cplx<T>{val[0], val[1]});
+        } else {
+            for (int e = 0; e < ne; ++e) gstore1(o + (full ? k : 2 * k + e), val[e]);
         }
         }
     };

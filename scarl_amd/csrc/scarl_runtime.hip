@@ -959,13 +959,24 @@ static void make_tiles(scarl_batch* b) {
     b->field_xtiles.assign(b->fields.size(), make_int2(0, 0));
     std::vector<std::vector<int>> members(1);
     int64_t acc = 0;
-    for (size_t i = 0; i < b->fields.size(); ++i) {
+    // fields of the same plans next to each other, so that a group holds few distinct plans and each
+    // of its launches covers many tiles (the order of the outputs does not change)
+    std::vector<int> order(b->fields.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
+    if (!b->global)
+        std::stable_sort(order.begin(), order.end(), [&](int u, int v) {
+            const FieldH &fu = b->fields[u].f, &fv = b->fields[v].f;
+            if (fu.plan_z != fv.plan_z) return fu.plan_z < fv.plan_z;
+            if (fu.plan_y != fv.plan_y) return fu.plan_y < fv.plan_y;
+            return fu.plan_x < fv.plan_x;
+        });
+    for (int i : order) {
         const FieldH& f = b->fields[i].f;
         if (!b->global && acc >= kGroupTiles) {
             members.emplace_back();
             acc = 0;
         }
-        members.back().push_back((int)i);
+        members.back().push_back(i);
         const int pz = f.n[2] > 1 ? f.plan_z : f.plan_y;
         acc += ((int64_t)(f.wp + 1) * (f.n[2] > 1 ? f.n[1] : f.n[2])) / b->ps.plans[pz].T + 1;
     }
