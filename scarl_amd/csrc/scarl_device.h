@@ -360,6 +360,59 @@ __device__ __forceinline__ void dft32(cplx<T>* v) {
     for (int i = 0; i < 32; ++i) v[i] = o[i];
 }
 
+// exp(+2 pi i m / R) for the composite codelets (R = 6, 9, 12; m < R), as literals
+template <int R, typename T>
+__device__ __forceinline__ cplx<T> unit_root(int m) {
+    constexpr double h = 0.86602540378443864676372317075293618;  // sin(60 deg)
+    if constexpr (R == 6) {
+        constexpr double c[6] = {1.0, 0.5, -0.5, -1.0, -0.5, 0.5};
+        constexpr double s[6] = {0.0, h, h, 0.0, -h, -h};
+        return {SCARL_C(c[m]), SCARL_C(s[m])};
+    } else if constexpr (R == 9) {
+        constexpr double c40 = 0.76604444311897803520239265055541667, s40 = 0.64278760968653932632264340990726343;
+        constexpr double c80 = 0.17364817766693034885171662676931480, s80 = 0.98480775301220805936674302458952301;
+        constexpr double c160 = -0.93969262078590838405410927732473146, s160 = 0.34202014332566873304409961468225958;
+        constexpr double c[9] = {1.0, c40, c80, -0.5, c160, c160, -0.5, c80, c40};
+        constexpr double s[9] = {0.0, s40, s80, h, s160, -s160, -h, -s80, -s40};
+        return {SCARL_C(c[m]), SCARL_C(s[m])};
+    } else {
+        static_assert(R == 12, "composite codelet roots");
+        constexpr double c[12] = {1.0, h, 0.5, 0.0, -0.5, -h, -1.0, -h, -0.5, 0.0, 0.5, h};
+        constexpr double s[12] = {0.0, 0.5, h, 1.0, h, 0.5, 0.0, -0.5, -h, -1.0, -h, -0.5};
+        return {SCARL_C(c[m]), SCARL_C(s[m])};
+    }
+}
+
+template <int R, typename T>
+__device__ __forceinline__ void dft(cplx<T>* v);
+
+// Backward DFT of R = R1 R2 points in registers (Cooley-Tukey): R2 DFTs of R1 over the inputs
+// n = R2 n1 + n2, twiddles exp(+2 pi i n2 k1 / R), then R1 DFTs of R2; output k1 + R1 k2.
+template <typename T, int R1, int R2>
+__device__ __forceinline__ void dft_ct(cplx<T>* v) {
+    constexpr int R = R1 * R2;
+    cplx<T> y[R2][R1];
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) {
+#pragma unroll
+        for (int n1 = 0; n1 < R1; ++n1) y[n2][n1] = v[R2 * n1 + n2];
+        dft<R1, T>(y[n2]);
+    }
+#pragma unroll
+    for (int n2 = 1; n2 < R2; ++n2)
+#pragma unroll
+        for (int k1 = 1; k1 < R1; ++k1) y[n2][k1] = cmul(y[n2][k1], unit_root<R, T>(n2 * k1));
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) {
+        cplx<T> b[R2];
+#pragma unroll
+        for (int n2 = 0; n2 < R2; ++n2) b[n2] = y[n2][k1];
+        dft<R2, T>(b);
+#pragma unroll
+        for (int k2 = 0; k2 < R2; ++k2) v[k1 + R1 * k2] = b[k2];
+    }
+}
+
 template <int R, typename T>
 __device__ __forceinline__ void dft(cplx<T>* v) {
     if constexpr (R == 2) dft2(v);
@@ -372,6 +425,9 @@ __device__ __forceinline__ void dft(cplx<T>* v) {
     else if constexpr (R == 13) dft_prime<T, 13>(v);
     else if constexpr (R == 16) dft16(v);
     else if constexpr (R == 32) dft32(v);
+    else if constexpr (R == 6) dft_ct<T, 2, 3>(v);
+    else if constexpr (R == 9) dft_ct<T, 3, 3>(v);
+    else if constexpr (R == 12) dft_ct<T, 4, 3>(v);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -417,10 +473,20 @@ __device__ __forceinline__ void stockham_stage(int N, int Ns, const cplx<T>* __r
     }
 }
 
-// Dispatch one stage on a runtime radix.  Radices above ECAP are never planned.
-template <int ECAP, typename T, class Load, class Store>
+// Dispatch one stage on a runtime radix.  Radices above ECAP are never planned; the composite
+// radices 6, 9 and 12 only exist in the COMP instantiations (their plans have their own kernels, so
+// that the other runtime-planned kernels do not carry their code).
+template <int ECAP, typename T, bool COMP = false, class Load, class Store>
 __device__ __forceinline__ void stage_dispatch(int R, int N, int Ns, const cplx<T>* __restrict__ tw, int b, int tpp,
                                                bool active, bool sync, Load&& ld, Store&& st) {
+    if constexpr (COMP) {
+        switch (R) {
+        case 6: stockham_stage<6, ECAP, T>(N, Ns, tw, b, tpp, active, sync, ld, st); return;
+        case 9: stockham_stage<9, ECAP, T>(N, Ns, tw, b, tpp, active, sync, ld, st); return;
+        case 12: stockham_stage<12, ECAP, T>(N, Ns, tw, b, tpp, active, sync, ld, st); return;
+        default: break;
+        }
+    }
     switch (R) {
     case 2: stockham_stage<2, ECAP, T>(N, Ns, tw, b, tpp, active, sync, ld, st); break;
     case 3: stockham_stage<3, ECAP, T>(N, Ns, tw, b, tpp, active, sync, ld, st); break;

@@ -30,8 +30,8 @@ struct DefaultStageIn {};
 
 // NS: the plan is a length with a prime factor above 13 (Bluestein, or the direct O(N^2) DFT); those
 // two paths live in their own instantiation so that they cost the Stockham kernels nothing.
-template <typename T, int LAYOUT, bool SIN, bool SOUT, int EC, bool NS = false, class Prep, class GLoad,
-          class GStore, class GIn = DefaultStageIn>
+template <typename T, int LAYOUT, bool SIN, bool SOUT, int EC, bool NS = false, bool COMP = false, class Prep,
+          class GLoad, class GStore, class GIn = DefaultStageIn>
 __device__ __forceinline__ void pencil_fft(const PlanDev& pl, const cplx<T>* __restrict__ pool, cplx<T>* lds,
                                            int npen, Prep&& prep, GLoad&& gld, GStore&& gst, GIn&& gin = GIn()) {
     const int tid = threadIdx.x;
@@ -138,23 +138,23 @@ __device__ __forceinline__ void pencil_fft(const PlanDev& pl, const cplx<T>* __r
     const int S = pl.nst;
     // stage 0
     if constexpr (SIN) {
-        if (S == 1 && !SOUT) stage_dispatch<EC, T>(pl.R[0], N, 1, pool, b, tpp, active, true, ld_l, st_g);
-        else stage_dispatch<EC, T>(pl.R[0], N, 1, pool, b, tpp, active, true, ld_l, st_l);
+        if (S == 1 && !SOUT) stage_dispatch<EC, T, COMP>(pl.R[0], N, 1, pool, b, tpp, active, true, ld_l, st_g);
+        else stage_dispatch<EC, T, COMP>(pl.R[0], N, 1, pool, b, tpp, active, true, ld_l, st_l);
     } else {
-        if (S == 1 && !SOUT) stage_dispatch<EC, T>(pl.R[0], N, 1, pool, b, tpp, active, false, ld_g, st_g);
-        else stage_dispatch<EC, T>(pl.R[0], N, 1, pool, b, tpp, active, false, ld_g, st_l);
+        if (S == 1 && !SOUT) stage_dispatch<EC, T, COMP>(pl.R[0], N, 1, pool, b, tpp, active, false, ld_g, st_g);
+        else stage_dispatch<EC, T, COMP>(pl.R[0], N, 1, pool, b, tpp, active, false, ld_g, st_l);
     }
     if (S > 1) {
         __syncthreads();
         for (int s = 1; s < S - 1; ++s) {
-            stage_dispatch<EC, T>(pl.R[s], N, pl.Ns[s], pool + pl.tw_off[s], b, tpp, active, true, ld_l, st_l);
+            stage_dispatch<EC, T, COMP>(pl.R[s], N, pl.Ns[s], pool + pl.tw_off[s], b, tpp, active, true, ld_l, st_l);
             __syncthreads();
         }
         const int s = S - 1;
         if constexpr (SOUT)
-            stage_dispatch<EC, T>(pl.R[s], N, pl.Ns[s], pool + pl.tw_off[s], b, tpp, active, true, ld_l, st_l);
+            stage_dispatch<EC, T, COMP>(pl.R[s], N, pl.Ns[s], pool + pl.tw_off[s], b, tpp, active, true, ld_l, st_l);
         else
-            stage_dispatch<EC, T>(pl.R[s], N, pl.Ns[s], pool + pl.tw_off[s], b, tpp, active, false, ld_l, st_g);
+            stage_dispatch<EC, T, COMP>(pl.R[s], N, pl.Ns[s], pool + pl.tw_off[s], b, tpp, active, false, ld_l, st_g);
     }
     if constexpr (SOUT) {
         __syncthreads();
@@ -386,6 +386,17 @@ struct GenericFft {
     __device__ __forceinline__ static void run(const PlanDev& pl, const cplx<T>* __restrict__ pool, cplx<T>* lds,
                                                int npen, A&&... a) {
         pencil_fft<T, LAYOUT, SIN, SOUT, EC, false>(pl, pool, lds, npen, static_cast<A&&>(a)...);
+    }
+};
+// runtime plans with a composite radix 6, 9 or 12 (kSpecGenericComp)
+template <int EC>
+struct GenericCompFft {
+    static constexpr bool kDirectGen = false;
+    static constexpr bool kXPair = false;
+    template <typename T, int LAYOUT, bool SIN, bool SOUT, class... A>
+    __device__ __forceinline__ static void run(const PlanDev& pl, const cplx<T>* __restrict__ pool, cplx<T>* lds,
+                                               int npen, A&&... a) {
+        pencil_fft<T, LAYOUT, SIN, SOUT, EC, false, true>(pl, pool, lds, npen, static_cast<A&&>(a)...);
     }
 };
 // lengths with a prime factor above 13 (kSpecNonSmooth): Bluestein or the direct DFT

@@ -117,7 +117,12 @@ struct Tile {
     X(15, 4, LAYOUT_BF, 32, 8, 4, 4)        \
     X(16, 8, LAYOUT_PF, 256, 16, 16, 16)    \
     X(17, 8, LAYOUT_BF, 256, 16, 16, 32)    \
-    X(18, 8, LAYOUT_BF, 128, 16, 8, 16)
+    X(18, 8, LAYOUT_BF, 128, 16, 8, 16)     \
+    X(19, 4, LAYOUT_PF, 144, 16, 9, 9)      \
+    X(20, 4, LAYOUT_PF, 192, 16, 12, 12)    \
+    X(21, 4, LAYOUT_BF, 72, 8, 9, 9)        \
+    X(22, 4, LAYOUT_PF, 96, 16, 6, 6)       \
+    X(23, 4, LAYOUT_BF, 144, 16, 9, 9)
 
 // The FP32 x pass of a 512-wide field (N = 256 = 16 x 16) on the matrix cores: split-precision
 // bf16 MFMA radix-16 stages (scarl_mfma.h), selected by SCARL_TUNE_MFMA_DFT16.
@@ -131,6 +136,8 @@ constexpr int kMfmaTileElems = 4096;
 // Runtime plans of lengths with a prime factor above 13: Bluestein (blu_M > 0) or the direct
 // O(N^2) DFT (nst = 0), run by their own kernel instantiation (NonSmoothFft in scarl_fft.h).
 constexpr int kSpecNonSmooth = 66;
+// Runtime plans with a composite radix 6, 9 or 12 (GenericCompFft in scarl_fft.h).
+constexpr int kSpecGenericComp = 67;
 
 // Blocks per field of the tile-partial reduction (k_mom_field); the per-field buffer holds
 // kMomChunks partials per field.

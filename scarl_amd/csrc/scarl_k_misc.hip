@@ -333,8 +333,8 @@ __global__ __launch_bounds__(kBlock) void k_lag_fft(const LagJob* __restrict__ j
     auto st_l = [&](int k, cplx<double> v) { L[k * Tc + p] = v; };
     for (int pass = 0; pass < 2; ++pass) {
         for (int s = 0; s < pl.nst; ++s) {
-            stage_dispatch<kEcap64, double>(pl.R[s], P, pl.Ns[s], pool + pl.tw_off[s], b, tpp, active, true, ld_l,
-                                            st_l);
+            stage_dispatch<kEcap64, double, true>(pl.R[s], P, pl.Ns[s], pool + pl.tw_off[s], b, tpp, active, true,
+                                                  ld_l, st_l);
             __syncthreads();
         }
         if (pass == 0) {

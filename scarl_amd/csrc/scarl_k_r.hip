@@ -252,6 +252,7 @@ static void r_dispatch(int spec, const FieldR& f, const PlanDev* plans, const cp
         SCARL_FIXED_SPECS(SCARL_CASE)
 #undef SCARL_CASE
     case kSpecNonSmooth: r_go<AXIS, NonSmoothFft<kEcap64>>(f, plans, p, nblocks, lds, s); break;
+    case kSpecGenericComp: r_go<AXIS, GenericCompFft<kEcap64>>(f, plans, p, nblocks, lds, s); break;
     default: r_go<AXIS, GenericFft<kEcap64>>(f, plans, p, nblocks, lds, s);
     }
 }

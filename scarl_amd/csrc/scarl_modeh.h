@@ -406,6 +406,11 @@ struct ModeHDispatch {
             else strided<NonSmoothFft<E1>>(axis, gen, fields, tiles, tile0, ntiles, plans, p, lds, s);
             return hipGetLastError();
         }
+        if (spec == kSpecGenericComp) {
+            if (ecap == E2) strided<GenericCompFft<E2>>(axis, gen, fields, tiles, tile0, ntiles, plans, p, lds, s);
+            else strided<GenericCompFft<E1>>(axis, gen, fields, tiles, tile0, ntiles, plans, p, lds, s);
+            return hipGetLastError();
+        }
         switch (spec) {
 #define SCARL_CASE(ID, P, L, N, R0, R1, TPP)                                                             \
     case ID:                                                                                             \
@@ -431,6 +436,9 @@ struct ModeHDispatch {
             if (spec == kSpecNonSmooth) {
                 if (ecap == E2) xpass<NonSmoothFft<E2>, true>(blend, fields, tiles, tile0, ntiles, plans, p, partials, lds, s);
                 else xpass<NonSmoothFft<E1>, true>(blend, fields, tiles, tile0, ntiles, plans, p, partials, lds, s);
+            } else if (spec == kSpecGenericComp) {
+                if (ecap == E2) xpass<GenericCompFft<E2>, true>(blend, fields, tiles, tile0, ntiles, plans, p, partials, lds, s);
+                else xpass<GenericCompFft<E1>, true>(blend, fields, tiles, tile0, ntiles, plans, p, partials, lds, s);
             } else if (ecap == E2) {
                 xpass<GenericFft<E2>, true>(blend, fields, tiles, tile0, ntiles, plans, p, partials, lds, s);
             } else {
@@ -441,6 +449,11 @@ struct ModeHDispatch {
         if (spec == kSpecNonSmooth) {
             if (ecap == E2) xpass<NonSmoothFft<E2>>(blend, fields, tiles, tile0, ntiles, plans, p, partials, lds, s);
             else xpass<NonSmoothFft<E1>>(blend, fields, tiles, tile0, ntiles, plans, p, partials, lds, s);
+            return hipGetLastError();
+        }
+        if (spec == kSpecGenericComp) {
+            if (ecap == E2) xpass<GenericCompFft<E2>>(blend, fields, tiles, tile0, ntiles, plans, p, partials, lds, s);
+            else xpass<GenericCompFft<E1>>(blend, fields, tiles, tile0, ntiles, plans, p, partials, lds, s);
             return hipGetLastError();
         }
         if (spec == kSpecMfma256) {

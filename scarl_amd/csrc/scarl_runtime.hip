@@ -326,13 +326,16 @@ struct PlanChoice {
     int order = 0;  // PV_GENERIC: 0 = largest radix first, 1 = smallest first
     int tdiv = 1;   // PV_GENERIC / PV_MFMA: pencils per tile divided by this
     int skew = 1;   // PV_GENERIC / PV_MFMA, butterfly-fastest layout: one LDS element of skew per 16
+    int comp = 0;   // PV_GENERIC: 1 = fewest stages with the composite radices 6, 9, 12 allowed
 };
 
 static std::string choice_name(const PlanChoice& c) {
     char b[96];
     switch (c.variant) {
     case PV_FIXED: snprintf(b, sizeof b, "fixed:%d", c.spec); break;
-    case PV_GENERIC: snprintf(b, sizeof b, "generic:e%d:o%d:t%d:s%d", c.ecap, c.order, c.tdiv, c.skew); break;
+    case PV_GENERIC:
+        snprintf(b, sizeof b, "generic:e%d:o%d:t%d:s%d:c%d", c.ecap, c.order, c.tdiv, c.skew, c.comp);
+        break;
     case PV_MFMA: snprintf(b, sizeof b, "mfma:t%d:s%d", c.tdiv, c.skew); break;
     case PV_DIRECT: snprintf(b, sizeof b, "direct"); break;
     default: snprintf(b, sizeof b, "default"); break;
@@ -348,7 +351,7 @@ static bool parse_choice(const char* txt, PlanChoice& c) {
         return true;
     }
     if (sscanf(txt, "fixed:%d", &c.spec) == 1) { c.variant = PV_FIXED; return true; }
-    if (sscanf(txt, "generic:e%d:o%d:t%d:s%d", &c.ecap, &c.order, &c.tdiv, &c.skew) == 4) {
+    if (sscanf(txt, "generic:e%d:o%d:t%d:s%d:c%d", &c.ecap, &c.order, &c.tdiv, &c.skew, &c.comp) == 5) {
         c.variant = PV_GENERIC;
         return (c.ecap == 16 || c.ecap == 32) && c.tdiv >= 1 && c.tdiv <= 8;
     }
@@ -439,6 +442,42 @@ struct PlanSet {
         return true;
     }
 
+    // Fewest Stockham stages over the radices 2 ... 16 (32 with ec >= 32) including the composite
+    // codelets 6, 9 and 12; among those, the least sum of per-point butterfly costs; largest first.
+    static bool factor_composite(int N, int ec, std::vector<int>& rad) {
+        rad.clear();
+        std::vector<int> R = {2, 3, 4, 5, 6, 7, 8, 9, 11, 12, 13, 16};
+        if (ec >= 32) R.push_back(32);
+        auto cost = [](int r) {  // real operations per point of one radix-r butterfly, roughly
+            switch (r) {
+            case 2: return 2.0; case 3: return 4.0; case 4: return 4.0; case 5: return 6.8; case 6: return 5.3;
+            case 7: return 9.0; case 8: return 5.3; case 9: return 7.0; case 11: return 13.0; case 12: return 6.0;
+            case 13: return 14.0; case 16: return 6.2; default: return 7.0;
+            }
+        };
+        std::vector<int> st(N + 1, 1 << 20), last(N + 1, 0);
+        std::vector<double> cs(N + 1, 1e30);
+        st[1] = 0;
+        cs[1] = 0.0;
+        for (int m = 2; m <= N; ++m) {
+            if (N % m) continue;
+            for (int r : R)
+                if (m % r == 0 && st[m / r] < (1 << 20)) {
+                    const int s2 = st[m / r] + 1;
+                    const double c2 = cs[m / r] + cost(r);
+                    if (s2 < st[m] || (s2 == st[m] && c2 < cs[m] - 1e-9)) {
+                        st[m] = s2;
+                        cs[m] = c2;
+                        last[m] = r;
+                    }
+                }
+        }
+        if (N < 2 || st[N] >= (1 << 20)) return false;
+        for (int m = N; m > 1; m /= last[m]) rad.push_back(last[m]);
+        std::sort(rad.begin(), rad.end(), [](int a, int b) { return a > b; });
+        return true;
+    }
+
     // the compile-time planned kernel for (N, precision, layout), or 0
     int fixed_spec(int N, int layout, int* r0, int* r1, int* tp) const {
         const int prec = fp64 ? 8 : 4;
@@ -453,6 +492,27 @@ struct PlanSet {
         SCARL_FIXED_SPECS(SCARL_MATCH)
 #undef SCARL_MATCH
         return spec;
+    }
+
+    // every compile-time planned kernel for (N, precision, layout)
+    std::vector<int> fixed_specs(int N, int layout) const {
+        const int prec = fp64 ? 8 : 4;
+        std::vector<int> ids;
+#define SCARL_ALL(ID, P, L, NN, A0, A1, TP) \
+        if (P == prec && L == layout && NN == N) ids.push_back(ID);
+        SCARL_FIXED_SPECS(SCARL_ALL)
+#undef SCARL_ALL
+        return ids;
+    }
+    static void fixed_spec_radices(int id, int* r0, int* r1, int* tp) {
+#define SCARL_RAD(ID, P, L, NN, A0, A1, TP) \
+        if (ID == id) {                     \
+            *r0 = A0;                       \
+            *r1 = A1;                       \
+            *tp = TP;                       \
+        }
+        SCARL_FIXED_SPECS(SCARL_RAD)
+#undef SCARL_RAD
     }
 
     // Stockham stages of radices rad with their twiddle tables; tpp from the register budget ec
@@ -514,7 +574,9 @@ struct PlanSet {
         if (variant == PV_DIRECT && factor(N, ecap, rad)) return undo(SCARL_E_UNSUPPORTED);
         if (variant == PV_FIXED) {
             const int spec = ch.variant == PV_FIXED ? ch.spec : fspec;
-            if (spec != fspec || !spec) return SCARL_E_UNSUPPORTED;
+            const std::vector<int> all = fixed_specs(N, layout);
+            if (!spec || std::find(all.begin(), all.end(), spec) == all.end()) return SCARL_E_UNSUPPORTED;
+            fixed_spec_radices(spec, &sr0, &sr1, &stpp);
             pl.spec = spec;
             if (ch.variant == PV_DEFAULT && mfma && !fp64 && layout == LAYOUT_BF && kind == 1 && N == 256 && sr0 == 16 &&
                 sr1 == 16 && stpp == 16)
@@ -544,12 +606,21 @@ struct PlanSet {
             stages(pl, rad, 16, true);
         } else if (N >= 2 && factor(N, ch.variant == PV_GENERIC ? ch.ecap : ecap, rad)) {
             const int ec = ch.variant == PV_GENERIC ? ch.ecap : ecap;
+            // the composite radices when asked for, and by default when they save a stage
+            std::vector<int> crad;
+            if (factor_composite(N, ec, crad) &&
+                ((ch.variant == PV_GENERIC && ch.comp) || (ch.variant == PV_DEFAULT && crad.size() < rad.size())))
+                rad = crad;
+            else if (ch.variant == PV_GENERIC && ch.comp)
+                return undo(SCARL_E_UNSUPPORTED);
             if (fp64 && ec != kEcap64) return undo(SCARL_E_UNSUPPORTED);
             if (ch.variant == PV_GENERIC) {
                 if (ch.order) std::reverse(rad.begin(), rad.end());
                 pl.skew = ch.skew;
             }
             pl.spec = 0;
+            for (int r : rad)
+                if (r == 6 || r == 9 || r == 12) pl.spec = kSpecGenericComp;
             pl.ecap = ec;
             stages(pl, rad, ec, false);
         } else if (N >= 32 && ch.variant != PV_DIRECT && ch.variant != PV_FIXED && ch.variant != PV_MFMA) {
@@ -615,7 +686,8 @@ struct PlanSet {
         // (compile-time planned kernels fix T = 256 / TPP; the spec table respects the bound)
         const size_t per = (size_t)(layout == LAYOUT_PF ? NL : pl.ldp) * csz;
         const int Tmax = (int)std::max<size_t>(1, (64 * 1024) / per);
-        const bool runtime_tile = pl.spec == 0 || pl.spec == kSpecMfmaGeneric || pl.spec == kSpecNonSmooth;
+        const bool runtime_tile = pl.spec == 0 || pl.spec == kSpecMfmaGeneric || pl.spec == kSpecNonSmooth ||
+                                  pl.spec == kSpecGenericComp;
         if (runtime_tile) pl.T = std::min(pl.T, Tmax);
         if (pl.spec == kSpecMfmaGeneric) pl.T = std::max(1, std::min(pl.T, kMfmaTileElems / N));
         if (runtime_tile) pl.T = std::max(1, pl.T / tdiv);
@@ -1220,9 +1292,7 @@ static void tune_file_append(const std::string& key, const PlanChoice& c) {
 
 static std::vector<PlanChoice> plan_candidates(const PlanSet& ps, int N, int layout) {
     std::vector<PlanChoice> c(1);  // the default rules first
-    int r0, r1, tp;
-    const int fs = ps.fixed_spec(N, layout, &r0, &r1, &tp);
-    if (fs) {
+    for (int fs : ps.fixed_specs(N, layout)) {
         PlanChoice f;
         f.variant = PV_FIXED;
         f.spec = fs;
@@ -1230,17 +1300,19 @@ static std::vector<PlanChoice> plan_candidates(const PlanSet& ps, int N, int lay
     }
     const std::vector<int> skews = layout == LAYOUT_BF ? std::vector<int>{1, 0} : std::vector<int>{1};
     for (int ec : ps.fp64 ? std::vector<int>{16} : std::vector<int>{16, 32})
-        for (int order : {0, 1})
-            for (int tdiv : {1, 2})
-                for (int sk : skews) {
-                    PlanChoice g;
-                    g.variant = PV_GENERIC;
-                    g.ecap = ec;
-                    g.order = order;
-                    g.tdiv = tdiv;
-                    g.skew = sk;
-                    c.push_back(g);
-                }
+        for (int comp : {0, 1})
+            for (int order : {0, 1})
+                for (int tdiv : {1, 2})
+                    for (int sk : skews) {
+                        PlanChoice g;
+                        g.variant = PV_GENERIC;
+                        g.ecap = ec;
+                        g.order = order;
+                        g.tdiv = tdiv;
+                        g.skew = sk;
+                        g.comp = comp;
+                        c.push_back(g);
+                    }
     {
         PlanChoice d;
         d.variant = PV_DIRECT;  // applies only to lengths with a prime factor above 13
@@ -1304,6 +1376,7 @@ static std::string plan_line(const PlanSet& ps, int i) {
     if (p.blu_M > 0) rad = "bluestein" + std::to_string(p.blu_M) + ":" + rad;
     const char* kern = p.spec == kSpecMfmaGeneric ? "mfma-generic" : p.spec == kSpecMfma256 ? "mfma-256"
                        : p.spec == kSpecNonSmooth ? (p.blu_M > 0 ? "bluestein" : "direct")
+                       : p.spec == kSpecGenericComp ? "generic-composite"
                        : p.spec ? "fixed" : "generic";
     snprintf(buf, sizeof buf, "pass=%s N=%d kernel=%s spec=%d radices=%s ecap=%d T=%d tpp=%d ldp=%d skew=%d choice=%s",
              role >= 0 && role < 3 ? roles[role] : "?", N, kern, p.spec, rad.c_str(), p.ecap, p.T, p.tpp, p.ldp,

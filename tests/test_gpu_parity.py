@@ -858,6 +858,72 @@ def test_bluestein_deck_size(hip):
     assert rel_l2(G, _blend_reference(gdim, starts, exts, spec)) <= FP64_TOL
 
 
+# Lengths planned with the composite radices 6, 9, 12 (108 = 12 x 9, 216 = 12 x 9 x 2) and the
+# compile-time kernels 144 = 16 x 9 (y/z and x half), 192 = 16 x 12, 72 = 8 x 9 (x half of 144)
+COMPOSITE_SHAPES = [(288, 108, 144), (144, 192, 216), (144, 144, 96)]
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+@pytest.mark.parametrize("case", range(len(COMPOSITE_SHAPES)))
+def test_composite_radices(hip, case, prec):
+    from scarl_amd import generator
+    n = COMPOSITE_SHAPES[case]
+    _say(f"composite radices n={n} fp{prec}")
+    L = (14.0, 10.0, 7.0)
+    spec = generator.FieldSpec(n=n, model=3, corrL=L, seed=12, stream=case, margi=2, avg=1.0, std_dev=0.3)
+    with generator.Batch.ensemble([spec], prec) as b:
+        b.run()
+        out = b.output(0)
+        rep = b.plan_report()
+    _say("  " + "; ".join(f"{r['pass']} N={r['N']} {r['kernel']} {r['radices']}" for r in rep))
+    if prec == 32:
+        assert any(r["kernel"] == "fixed" and ("16x9" in r["radices"] or "16x12" in r["radices"] or
+                                               "8x9" in r["radices"]) for r in rep)
+    ref = O.gen_mode_h(n, (1, 1, 1), 3, L, seed=12, stream=case, margi=2, avg=1.0, std=0.3)
+    assert rel_l2(out, ref) <= (FP64_TOL if prec == 64 else FP32_TOL)
+
+
+# (shape, {plan key: choice}) written into a SCARL_TUNE_CACHE file: the batch then uses exactly these
+# plans, so candidates the autotuner would not pick are checked too
+FORCED = [((192, 96, 144), {"fp32/144/0/0/1": "fixed:19", "fp32/96/0/0/0": "fixed:22", "fp32/96/1/1/2": "fixed:11"}),
+          ((192, 96, 144), {"fp32/144/0/0/1": "generic:e16:o1:t2:s1:c1", "fp32/96/0/0/0": "generic:e32:o0:t1:s1:c0",
+                            "fp32/96/1/1/2": "generic:e16:o0:t1:s0:c1"}),
+          ((144, 216, 80), {"fp32/80/0/0/1": "mfma:t2:s1", "fp32/216/0/0/0": "generic:e16:o0:t2:s1:c1",
+                            "fp32/72/1/1/2": "fixed:21"}),
+          ((144, 216, 72), {"fp64/72/0/0/1": "generic:e16:o1:t1:s1:c1", "fp64/216/0/0/0": "generic:e16:o0:t2:s1:c1",
+                            "fp64/72/1/1/2": "generic:e16:o1:t2:s0:c1"})]
+
+
+@pytest.mark.parametrize("case", range(len(FORCED)))
+def test_forced_plan_choices(hip, tmp_path, case):
+    import subprocess
+    import sys
+    from conftest import ROOT
+    n, picks = FORCED[case]
+    prec = 64 if next(iter(picks)).startswith("fp64") else 32
+    cache = tmp_path / "tune.txt"
+    cache.write_text("".join(f"{k} {v}\n" for k, v in picks.items()))
+    code = (
+        "import numpy as np\n"
+        "from scarl_amd import _capi as A, generator as G\n"
+        "from oracle import oracle as O\n"
+        "A.call('scarl_hip_init', 0)\n"
+        "G.set_tuning(A.TUNE_AUTOTUNE, 1)\n"
+        f"sp = G.FieldSpec(n={n!r}, model=1, corrL=(9.0, 8.0, 6.0), seed=3, margi=2, avg=1.0, std_dev=0.3)\n"
+        f"b = G.Batch.ensemble([sp], {prec})\n"
+        "b.run(); f = b.output(0)\n"
+        "print(';'.join(r['choice'] + ('/cached' if r.get('cached') else '') for r in b.plan_report()))\n"
+        f"ref = O.gen_mode_h({n!r}, (1, 1, 1), 1, (9.0, 8.0, 6.0), seed=3, margi=2, avg=1.0, std=0.3)\n"
+        "print(float(np.linalg.norm(f - ref) / np.linalg.norm(ref)))\n")
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, cwd=ROOT,
+                       env=dict(os.environ, SCARL_TUNE_CACHE=str(cache)), timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    used, err = r.stdout.strip().splitlines()[-2:]
+    _say(f"  forced {used} rel L2 {err}")
+    assert sorted(c.split("/")[0] for c in used.split(";")) == sorted(picks.values())
+    assert float(err) <= (FP64_TOL if prec == 64 else FP32_TOL)
+
+
 def test_deck_geometry_blend(hip):
     """test/3_TEST sample 1 (x in [0, 10], corrL 1, overlap 1, 5 points per corrL, Gaussian,
     lognormal 1/1, seed 1) on a 2x2x2 topology: the eight ranks' subdomains (23^3, overlap 5,
