@@ -10,18 +10,23 @@
 
 namespace scarl {
 
-__device__ __forceinline__ double mode_r_amp(const FieldR& f, int i, int j, int k) {
-    // SRC_F/fftw_ScaRL.f90:305-370 (kPoints, Sk product in axis order, sqrt(S) cos(2 PI phi))
+// exp(-PI k_d^2) of the reference's Gaussian spectrum along axis d at octant index i
+__device__ __forceinline__ double mode_r_gauss(const FieldR& f, int d, int i) {
+    const double k = ((double)i * f.dk[d]) / f.corrL[d];
+    return exp(-kPiRef * (k * k));
+}
+
+// SRC_F/fftw_ScaRL.f90:305-370 (kPoints, Sk product in axis order, sqrt(S) cos(2 PI phi)); e1, e2:
+// mode_r_gauss along y and z, which are constant along an x pencil and evaluated once per pencil;
+// the product keeps the reference's order S = ((corrL1 e0) corrL2 e1) corrL3 e2
+__device__ __forceinline__ double mode_r_amp(const FieldR& f, int i, int j, int k, double e1, double e2) {
     const int64_t pidx = i + (int64_t)f.m[0] * (j + (int64_t)f.m[1] * k);
     const double ph = (f.src_mode == SRC_HOST_NOISE) ? gload1(f.phi + pidx)
                                                      : uniform53(static_cast<uint64_t>(pidx), f.seed, f.stream);
-    const double k0 = ((double)i * f.dk[0]) / f.corrL[0];
-    const double k1 = ((double)j * f.dk[1]) / f.corrL[1];
-    const double k2 = ((double)k * f.dk[2]) / f.corrL[2];
     double S = 1.0;
-    S = S * f.corrL[0] * exp(-kPiRef * (k0 * k0));
-    S = S * f.corrL[1] * exp(-kPiRef * (k1 * k1));
-    S = S * f.corrL[2] * exp(-kPiRef * (k2 * k2));
+    S = S * f.corrL[0] * mode_r_gauss(f, 0, i);
+    S = S * f.corrL[1] * e1;
+    S = S * f.corrL[2] * e2;
     return 1.0 * sqrt(S) * cos(2.0 * kPiRef * ph);
 }
 
@@ -49,6 +54,7 @@ __global__ __launch_bounds__(kBlock) void k_r_pass(const FieldR fp, const PlanDe
     struct Ctx {
         int64_t in[2], out[2];   // element-0 offsets of input / output of each real pencil
         int jy[2], kz[2];        // x pass: octant coordinates of the rows
+        double e1[2], e2[2];     // x pass: the rows' y and z spectrum factors
         bool hasb;
     };
     auto prep = [&](int p) -> Ctx {
@@ -60,15 +66,19 @@ __global__ __launch_bounds__(kBlock) void k_r_pass(const FieldR fp, const PlanDe
             if constexpr (AXIS == 0) {
                 c.jy[e] = (int)(P % m2);
                 c.kz[e] = (int)(P / m2);
+                c.e1[e] = mode_r_gauss(f, 1, c.jy[e]);
+                c.e2[e] = mode_r_gauss(f, 2, c.kz[e]);
                 c.in[e] = 0;
                 c.out[e] = (int64_t)n1 * P;
             } else if constexpr (AXIS == 1) {
                 const int64_t i = P % n1, k = P / n1;
                 c.jy[e] = c.kz[e] = 0;
+                c.e1[e] = c.e2[e] = 0.0;
                 c.in[e] = i + (int64_t)n1 * m2 * k;
                 c.out[e] = i + (int64_t)n1 * n2 * k;
             } else {
                 c.jy[e] = c.kz[e] = 0;
+                c.e1[e] = c.e2[e] = 0.0;
                 c.in[e] = P;
                 c.out[e] = P;
             }
@@ -77,7 +87,7 @@ __global__ __launch_bounds__(kBlock) void k_r_pass(const FieldR fp, const PlanDe
     };
     auto rin = [&](const Ctx& c, int e, int j) -> double {
         if (j >= m) return 0.0;
-        if constexpr (AXIS == 0) return mode_r_amp(f, j, c.jy[e], c.kz[e]);
+        if constexpr (AXIS == 0) return mode_r_amp(f, j, c.jy[e], c.kz[e], c.e1[e], c.e2[e]);
         else if constexpr (AXIS == 1) return gload1(f.tmp1 + (c.in[e] + (int64_t)n1 * j));
         else return gload1(f.tmp2 + (c.in[e] + (int64_t)n1 * n2 * j));
     };
@@ -111,7 +121,31 @@ __global__ __launch_bounds__(kBlock) void k_r_pass(const FieldR fp, const PlanDe
         rout(c, 0, o, v.x);
         if (c.hasb) rout(c, 1, o, v.y);
     };
-    if constexpr (AXIS == 0) Fft::template run<double, LAYOUT_BF, true, false>(pl, pool, lds, npen, prep, gld, gst);
+    // x pass: each octant amplitude X_j feeds V_j and V_{n-j}; one thread handles the pair (j, n-j)
+    // and generates each amplitude once (gld would generate it for both)
+    auto gin = [&](const Ctx& c, int b0, int tstep, auto&& put) {
+        auto vpair = [&](int j, double xa, double xa2, double xb, double xb2) -> cplx<double> {
+            const cplx<double> w = pre[j];
+            const cplx<double> va = {xa * w.x + xa2 * w.y, xa * w.y - xa2 * w.x};
+            const cplx<double> vb = {xb * w.x + xb2 * w.y, xb * w.y - xb2 * w.x};
+            return {va.x - vb.y, va.y + vb.x};
+        };
+#pragma unroll 1
+        for (int j = b0; 2 * j <= n; j += tstep) {
+            const double xa = rin(c, 0, j), xb = c.hasb ? rin(c, 1, j) : 0.0;
+            if (j == 0) {
+                put(0, vpair(0, xa, 0.0, xb, 0.0));  // X_n = 0
+            } else if (2 * j == n) {
+                put(j, vpair(j, xa, xa, xb, xb));
+            } else {
+                const double ya = rin(c, 0, n - j), yb = c.hasb ? rin(c, 1, n - j) : 0.0;
+                put(j, vpair(j, xa, ya, xb, yb));
+                put(n - j, vpair(n - j, ya, xa, yb, xb));
+            }
+        }
+    };
+    if constexpr (AXIS == 0)
+        Fft::template run<double, LAYOUT_BF, true, false>(pl, pool, lds, npen, prep, gld, gst, gin);
     else Fft::template run<double, LAYOUT_PF, false, false>(pl, pool, lds, npen, prep, gld, gst);
 }
 
