@@ -478,6 +478,8 @@ def main():
     ap.add_argument("--no-configs", action="store_true", help="skip the config 1, 4 and 5 lines")
     ap.add_argument("--no-tune", action="store_true", help="headline with the planner's fixed rules")
     ap.add_argument("--probe-only", action="store_true", help="run the copy probe and the headline only")
+    ap.add_argument("--only", choices=("config1", "config4", "config5"),
+                    help="run one config line alone (for profiling) and print it")
     ap.add_argument("--traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="write profiles/traffic.json from two rocprofv3 --pmc counter CSVs and exit")
     args = ap.parse_args()
@@ -493,6 +495,10 @@ def main():
     from scarl_amd import generator as Gm
 
     A.call("scarl_hip_init", 0)
+    if args.only:
+        fn = {"config1": config1, "config4": config4, "config5": config5}[args.only]
+        print(json.dumps({args.only: fn(args)}), flush=True)
+        return
     lib = A.lib()
     stream = C.c_void_p()
     A.call("scarl_hip_get_stream", C.byref(stream))
