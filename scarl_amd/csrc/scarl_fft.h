@@ -221,7 +221,10 @@ __device__ __forceinline__ void pencil_fft_fixed(const PlanDev& pl, const cplx<T
         b = tid % TPP;
         p = tid / TPP;
     }
-    const bool active = p < npen;
+    // When TPP does not divide the block (TT TPP < 256) the last threads get b = TPP (pencil-fastest)
+    // or p = TT (butterfly-fastest): neither is a thread of the plan.  Had they run, a butterfly
+    // j = TPP would read element N of its pencil, one past its end.
+    const bool active = p < npen && (LAYOUT != LAYOUT_PF || b < TPP);
     auto addr = [&](int n) -> int {
         if constexpr (LAYOUT == LAYOUT_PF) return n * TT + p;
         else return p * LDP + n + (n >> 4);
@@ -288,7 +291,7 @@ __device__ __forceinline__ void pencil_fft_fixed_gen(const PlanDev& pl, const cp
     constexpr int BMAX = (nb + TPP - 1) / TPP;
     const int tid = threadIdx.x;
     const int p = tid % TT, b = tid / TT;
-    const bool active = p < npen;
+    const bool active = p < npen && b < TPP;  // b = TPP: a thread past TT TPP (TPP not dividing 256)
     const auto ctx = prep(active ? p : 0);
     auto ld_l = [&](int n) -> cplx<T> { return lds[n * TT + p]; };
     auto st_l = [&](int n, cplx<T> v) { lds[n * TT + p] = v; };
@@ -484,6 +487,24 @@ __device__ __forceinline__ T pou_sqrt_weight(int i, int n, int nlo, int nhi) {
         const T t = T(i - (n - nhi)) / T(nhi - 1);
         if constexpr (sizeof(T) == 4) w *= 0.5f * (1.0f + cospif(t));
         else w *= 0.5 * (1.0 + cos(kPiRef * t));
+    }
+    return w;
+}
+
+// Square root of pou_sqrt_weight's product: sqrt(0.5 (1 + cos(pi t))) = |cos(pi t / 2)| for each
+// ramp, so one cosine per ramp and no square root (the same angles, the reference's PI in FP64).
+template <typename T>
+__device__ __forceinline__ T pou_sqrt_factor(int i, int n, int nlo, int nhi) {
+    T w = T(1);
+    if (nlo >= 2 && i < nlo) {
+        const T t = T(nlo - 1 - i) / T(nlo - 1);
+        if constexpr (sizeof(T) == 4) w *= fabsf(cospif(0.5f * t));
+        else w *= fabs(cos(0.5 * kPiRef * t));
+    }
+    if (nhi >= 2 && i >= n - nhi) {
+        const T t = T(i - (n - nhi)) / T(nhi - 1);
+        if constexpr (sizeof(T) == 4) w *= fabsf(cospif(0.5f * t));
+        else w *= fabs(cos(0.5 * kPiRef * t));
     }
     return w;
 }

@@ -92,8 +92,11 @@ __global__ __launch_bounds__(kBlock) void k_h_strided(const FieldH* __restrict__
     const PlanDev& pl = plans[axis == 2 ? f.plan_z : f.plan_y];
     const int wp = f.wp, n2 = f.n[1], n3 = f.n[2];
     const int nother = (axis == 2) ? n2 : n3;
-    const bool nyq = tl.nyq != 0;
-    const int npen = min(pl.T, (nyq ? nother : wp * nother) - tl.pencil0);
+    const bool nyq = tl.kind == TILE_NYQ;
+    const int npen = min(tl.npen > 0 ? tl.npen : pl.T, (tl.kind == TILE_ROWS     ? wp * nother
+                                                        : tl.kind == TILE_OFFKX0 ? (wp - 1) * nother
+                                                                                 : nother) -
+                                                           tl.pencil0);
     const int64_t wz = f.wz;
     const int64_t S = nyq ? ((axis == 2) ? (int64_t)n2 : 1) : ((axis == 2) ? wz : (int64_t)wp);
     cplx<T>* ws = reinterpret_cast<cplx<T>*>(f.ws) + (nyq ? wz * n3 : 0);
@@ -105,8 +108,18 @@ __global__ __launch_bounds__(kBlock) void k_h_strided(const FieldH* __restrict__
     auto prep = [&](int p) -> Ctx {
         const int P = tl.pencil0 + p;
         if (nyq) return {(axis == 2) ? (int64_t)P : (int64_t)n2 * P, wp, P};
-        const int kx = P % wp, q = P / wp;
-        const int64_t base = (axis == 2) ? (int64_t)P : kx + wz * q;
+        int kx, q;
+        if (tl.kind == TILE_KX0) {
+            kx = 0;
+            q = P;
+        } else if (tl.kind == TILE_OFFKX0) {
+            kx = 1 + P % (wp - 1);
+            q = P / (wp - 1);
+        } else {
+            kx = P % wp;
+            q = P / wp;
+        }
+        const int64_t base = (axis == 2) ? kx + (int64_t)wp * q : kx + wz * q;
         return {base, kx, q};
     };
     auto gld = [&](const Ctx& c, int n) -> cplx<T> {
@@ -174,14 +187,15 @@ __global__ __launch_bounds__(kBlock) void k_h_strided(const FieldH* __restrict__
     };
     if constexpr (GEN) {
         if (axis == 2) {
-            // Tiles whose pencils are all off the self-conjugate planes (no kx = 0 pencil, not the
-            // Nyquist plane) with device noise: the first stage's butterflies hold the pairs
+            // Tiles whose pencils are all off the self-conjugate planes (TILE_OFFKX0, or a row
+            // range without kx = 0) with device noise: the first stage's butterflies hold the pairs
             // (q, q + N/2), generated straight into registers. The test is uniform over the tile,
-            // so only the paired draw is unrolled into the butterflies. Other tiles stage the
-            // generated pencils through LDS.
+            // so only the paired draw is unrolled into the butterflies. Other tiles (kx = 0, the
+            // Nyquist plane, host noise) stage the generated pencils through LDS.
             const int kx_first = tl.pencil0 % wp;
-            const bool direct = gen && !nyq && f.src_mode == SRC_DEVICE_NOISE && kx_first != 0 &&
-                                kx_first + npen <= wp;
+            const bool off_sc = tl.kind == TILE_OFFKX0 ||
+                                (tl.kind == TILE_ROWS && kx_first != 0 && kx_first + npen <= wp);
+            const bool direct = gen && f.src_mode == SRC_DEVICE_NOISE && off_sc;
             if constexpr (Fft::kDirectGen) {
                 if (direct) {
                     Fft::template run_gen<T>(pl, pool, lds, npen, prep, gst, GenPair<decltype(gpair_dev)>{gpair_dev});
@@ -226,7 +240,9 @@ __global__ __launch_bounds__(kBlock) void k_h_xpass(const FieldH* __restrict__ f
     cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem);
     const int tix = tile0 + blockIdx.x;
     const Tile tl = tiles[tix];
-    const FieldH& f = fields[tl.field];
+    // a copy: the compiler then keeps the descriptor in registers across the epilogue's stores
+    // instead of reloading it after each (they could alias it as far as it can tell)
+    const FieldH f = fields[tl.field];
     const PlanDev& pl = plans[f.plan_x];
     const int n1 = f.n[0], n2 = f.n[1], n3 = f.n[2], M = n1 / 2, h1 = f.h1;
     constexpr bool full = FULL;  // odd n1 (full-length C2C over the Hermitian extension)
@@ -243,8 +259,20 @@ __global__ __launch_bounds__(kBlock) void k_h_xpass(const FieldH* __restrict__ f
         const cplx<T>* X;   // row (kx < wp)
         const cplx<T>* Xn;  // its kx = M element in the Nyquist plane (even n1)
         T* o;
-        T wyz;     // PoU weight of the row (blend)
-        int myz;   // overlap mask bits of the row (blend)
+        T swyz;    // square root of the row's PoU weight (blend)
+        int myz;   // overlap mask bits of the row (blend): 2 y, 4 z
+    };
+    // blend: the x factors of the PoU weight's square root for the tile's rows (all of one field),
+    // in LDS after the transform's tile (1 off the ramps)
+    T* wxl = reinterpret_cast<T*>(smem + (size_t)pl.lds_elems * sizeof(cplx<T>));
+    if constexpr (BLEND) {
+#pragma unroll 1
+        for (int i = threadIdx.x; i < n1; i += kBlock) wxl[i] = pou_sqrt_factor<T>(i, n1, f.nov_lo[0], f.nov_hi[0]);
+    }
+    auto row_mask = [&](int row) {
+        const int ky = row % n2, kz = row / n2;
+        return (in_overlap(ky, n2, f.nov_lo[1], f.nov_hi[1]) ? 2 : 0) |
+               (in_overlap(kz, n3, f.nov_lo[2], f.nov_hi[2]) ? 4 : 0);
     };
     auto prep = [&](int p) -> Ctx {
         const int row = tl.pencil0 + p;
@@ -252,10 +280,9 @@ __global__ __launch_bounds__(kBlock) void k_h_xpass(const FieldH* __restrict__ f
         Ctx c = {ws + (int64_t)ky * f.wp + kz * f.wz, ws + f.wz * n3 + row, out + ky * f.out_ld1 + kz * f.out_ld2,
                  T(1), 0};
         if (BLEND) {
-            c.wyz = pou_sqrt_weight<T>(ky, n2, f.nov_lo[1], f.nov_hi[1]) *
-                    pou_sqrt_weight<T>(kz, n3, f.nov_lo[2], f.nov_hi[2]);
-            c.myz = (in_overlap(ky, n2, f.nov_lo[1], f.nov_hi[1]) ? 2 : 0) |
-                    (in_overlap(kz, n3, f.nov_lo[2], f.nov_hi[2]) ? 4 : 0);
+            c.swyz = pou_sqrt_factor<T>(ky, n2, f.nov_lo[1], f.nov_hi[1]) *
+                     pou_sqrt_factor<T>(kz, n3, f.nov_lo[2], f.nov_hi[2]);
+            c.myz = row_mask(row);
         }
         return c;
     };
@@ -302,38 +329,30 @@ __global__ __launch_bounds__(kBlock) void k_h_xpass(const FieldH* __restrict__ f
             }
             gstore(reinterpret_cast<cplx<T>*>(o + 2 * k), cplx<T>{r0, r1});
         } else {
-        // Outside every overlap (mask 0) a point has one owner: no weight, no read, marginal and
-        // moments at once.  Even n1: the two outputs of z go out as one 2-element store when neither
-        // reads what an earlier owner wrote and the row is aligned for it.
-        const T wyz = c.wyz;
-        const int myz = c.myz;
+        // Branch-light: every point takes the weight (1 off the ramps: the row's y/z factor times
+        // the x factor from the tile's LDS table), a point reads what an earlier owner wrote only
+        // if it is not its first owner, and the marginal and the moments count only at its last
+        // owner (a zero contribution elsewhere).  Even n1: both outputs of z go out as one
+        // 2-element store when the row is aligned for it.
         const int ne = full ? 1 : 2;
         T val[2];
-        bool rd[2];
 #pragma unroll
         for (int e = 0; e < 2; ++e) {
             if (e >= ne) break;
             const int i = full ? k : 2 * k + e;
-            const bool ox = in_overlap(i, n1, f.nov_lo[0], f.nov_hi[0]);
-            const int msk = myz | (ox ? 1 : 0);
-            T v = e ? z.y : z.x;
-            if (msk) {
-                const T w = ox ? wyz * pou_sqrt_weight<T>(i, n1, f.nov_lo[0], f.nov_hi[0]) : wyz;
-                v *= sqrt(w);
+            const int msk = c.myz | (in_overlap(i, n1, f.nov_lo[0], f.nov_hi[0]) ? 1 : 0);
+            const T v0 = (e ? z.y : z.x) * (c.swyz * wxl[i]);
+            const T v = (f.color & msk) ? v0 + gload1(o + i) : v0;  // not the first owner: add
+            const bool last = (f.color & msk) == msk;
+            const T r = apply_margi<T>(v, f.margi, mu, sd);
+            val[e] = last ? r : v;
+            if (mom) {
+                const double d = last ? (double)r : 0.0;
+                ms += d;
+                ms2 += d * d;
             }
-            rd[e] = (f.color & msk) != 0;  // not the first owner: add to what is there
-            if (rd[e]) v += gload1(o + i);
-            if ((f.color & msk) == msk) {  // the last owner
-                v = apply_margi<T>(v, f.margi, mu, sd);
-                if (mom) {
-                    const double d = v;
-                    ms += d;
-                    ms2 += d * d;
-                }
-            }
-            val[e] = v;
         }
-        if (!full && !rd[0] && !rd[1] && (reinterpret_cast<uintptr_t>(o) % (2 * sizeof(T))) == 0) {
+        if (!full && (reinterpret_cast<uintptr_t>(o) % (2 * sizeof(T))) == 0) {
             gstore(reinterpret_cast<cplx<T>*>(o + 2 * k), cplx<T>{val[0], val[1]});
         } else {
             for (int e = 0; e < ne; ++e) gstore1(o + (full ? k : 2 * k + e), val[e]);

@@ -1002,15 +1002,29 @@ static int finish_amplitudes(scarl_batch* b) {
 // regular tiles that hold a kx = 0 pencil (kind 2) and the other regular tiles of the workspace
 // rows (kind 0).  The first two carry the self-conjugate noise, whose unpaired draws make them
 // costlier in the generating pass.
-static void strided_tiles(int field, const FieldH& f, int nother, int T, int kind, std::vector<Tile>& out) {
-    if (kind == 1) {
+static void strided_tiles(int field, const FieldH& f, int nother, int T, int sel, std::vector<Tile>& out) {
+    if (sel == 1) {
         if (f.nyq)
-            for (int p = 0; p < nother; p += T) out.push_back({field, p, 1, 0});
+            for (int p = 0; p < nother; p += T) out.push_back({field, p, TILE_NYQ, 0});
+        return;
+    }
+    if (sel == 3) {
+        for (int p = 0; p < nother; p += T) out.push_back({field, p, TILE_KX0, 0});
+        return;
+    }
+    if (sel == 4) {
+        if (T >= 2 && f.wp % T == 0) {
+            // rows of a multiple of T: tiles aligned to T in kx, the first of a row from kx = 1
+            for (int64_t p = 0; p < (int64_t)f.wp * nother; p += T)
+                out.push_back(p % f.wp == 0 ? Tile{field, (int)p + 1, TILE_ROWS, T - 1} : Tile{field, (int)p, TILE_ROWS, 0});
+        } else {
+            for (int64_t p = 0; p < (int64_t)(f.wp - 1) * nother; p += T) out.push_back({field, (int)p, TILE_OFFKX0, 0});
+        }
         return;
     }
     for (int64_t p = 0; p < (int64_t)f.wp * nother; p += T) {
         const bool has_kx0 = (p % f.wp == 0) || (p / f.wp != (p + std::min<int64_t>(T, (int64_t)f.wp * nother - p) - 1) / f.wp);
-        if (has_kx0 == (kind == 2)) out.push_back({field, (int)p, 0, 0});
+        if (has_kx0 == (sel == 2)) out.push_back({field, (int)p, TILE_ROWS, 0});
     }
 }
 
@@ -1061,9 +1075,21 @@ static void make_tiles(scarl_batch* b) {
             yp[f.plan_y].push_back(i);
         }
         for (auto& kv : zp) {
+            // The generating z pass, Nyquist tiles first, then the tiles holding a kx = 0 pencil
+            // (which stage their noise through LDS), then the rest (which draw it straight into
+            // the first stage).  Rows of at least 8 aligned tiles keep the row tiles (one slow
+            // tile in 8 or fewer; measured on 512^3 fields, the kx = 0 column tiles with their
+            // split 128-byte segments cost the y and x passes more than they save); shorter or
+            // unaligned rows take kx = 0 column tiles and off-kx = 0 tiles, so that nearly every
+            // tile draws directly (config 5's 64^3 rows are one tile each).
             const int t0 = (int)b->tz.size(), T = b->ps.plans[kv.first].T;
-            for (int kind : {1, 2, 0})
-                for (int i : kv.second) strided_tiles(i, b->fields[i].f, b->fields[i].f.n[1], T, kind, b->tz);
+            auto rows = [&](const FieldH& f) { return T >= 2 && f.wp % T == 0 && f.wp >= 8 * T; };
+            for (int phase = 0; phase < 3; ++phase)
+                for (int i : kv.second) {
+                    const FieldH& f = b->fields[i].f;
+                    const int sel = phase == 0 ? 1 : rows(f) ? (phase == 1 ? 2 : 0) : (phase == 1 ? 3 : 4);
+                    strided_tiles(i, f, f.n[1], T, sel, b->tz);
+                }
             g.cz.push_back({kv.first, t0, (int)b->tz.size()});
         }
         for (auto& kv : yp) {
@@ -1206,10 +1232,13 @@ static int enqueue_group(scarl_batch* b, const scarl_batch::Group& g, hipStream_
     for (const auto& cl : g.cx) {
         if (only_plan >= 0 && cl.plan != only_plan) continue;
         const PlanDev& pl = b->ps.plans[cl.plan];
+        // x-pass plans of odd n1 are plain C2C (no pre-twiddle table); a blend keeps its x weights
+        // (n1 reals) in LDS after the transform's tile
+        const bool full = pl.pre_off < 0;
+        const size_t lds = (size_t)pl.lds_elems * csz + (b->global ? (size_t)(full ? pl.N : 2 * pl.N) * (csz / 2) : 0);
         SCARL_TRY(timed("h_xpass", [&] {
-            // x-pass plans of odd n1 are plain C2C (no pre-twiddle table)
-            return launch_h_xpass(f64, pl.spec, pl.ecap, b->global, pl.pre_off < 0, b->d_fields, b->d_tx, cl.t0, cl.t1 - cl.t0,
-                                  b->ps.d_plans, b->ps.d_pool, b->d_partials, (size_t)pl.lds_elems * csz, s);
+            return launch_h_xpass(f64, pl.spec, pl.ecap, b->global, full, b->d_fields, b->d_tx, cl.t0, cl.t1 - cl.t0,
+                                  b->ps.d_plans, b->ps.d_pool, b->d_partials, lds, s);
         }));
     }
     return 0;

@@ -246,6 +246,21 @@ def test_hermitian_device_noise_generated_in_first_stage(hip, prec, n):
     assert rel_l2(generator.gen_field_h(spec, prec), ref) <= (FP32_TOL if prec == 32 else FP64_TOL)
 
 
+@pytest.mark.parametrize("prec,n", [(32, (64, 10, 64)), (32, (256, 6, 128)), (32, (512, 4, 64)), (32, (40, 6, 96)),
+                                    (32, (33, 7, 160)), (64, (64, 5, 64)), (64, (42, 5, 96)), (32, (2, 9, 64))])
+def test_z_tile_kinds(hip, prec, n):
+    """The generating z pass's tile kinds: rows of one or a few aligned tiles (the first tile of a
+    row from kx = 1, the kx = 0 pencils in column tiles), rows of 8 or more aligned tiles (row tiles,
+    the first holding kx = 0), and rows that are not a multiple of the tile (tiles over the pencils
+    off kx = 0, straddling rows), even and odd n1, with device noise against the oracle."""
+    from scarl_amd import generator
+    _say(f"z tile kinds fp{prec} n={n}")
+    L = (6.0, 3.0, 10.0)
+    spec = generator.FieldSpec(n=n, model=3, corrL=L, seed=29, stream=2, margi=2, avg=1.0, std_dev=0.3)
+    ref = O.gen_mode_h(n, (1, 1, 1), 3, L, seed=29, stream=2, margi=2, avg=1.0, std=0.3)
+    assert rel_l2(generator.gen_field_h(spec, prec), ref) <= (FP32_TOL if prec == 32 else FP64_TOL)
+
+
 @pytest.mark.parametrize("n", [(23, 12, 10), (41, 41, 41), (81, 30, 17), (5, 6, 1), (45, 33, 1)])
 def test_hermitian_odd_x_extent(hip, n):
     """Odd x extents (the reference geometry's L = 1 + ceiling(...) is usually odd): rows of
@@ -597,6 +612,40 @@ def test_config1_2d_gaussian_against_numpy_irfftn(hip):
     assert rel_l2(gpu[:, :, 0], ref) <= FP64_TOL
     _say(f"config 1: mean {gpu.mean():.4f} variance {gpu.var():.4f}")
     assert abs(gpu.mean()) < 0.2 and abs(gpu.var() - 1.0) < 0.3  # ~ (256/8)^2 = 1024 correlation areas
+
+
+def test_blend_without_overlap_equals_ensemble(hip):
+    """Regression: 8x8x2 subdomains of 112-144 points tiling 1024x1024x256 without overlap. The
+    workspace slab is exactly 520 x 2 MiB, and the last subdomain (144^3) runs the 144-point y pass
+    (9 threads per pencil, 28 pencils: four threads of each block fall outside the plan).  Those
+    threads once ran a butterfly past the pencil, reading one element beyond the slab's end, which
+    here is an unmapped page.  With no overlap the blend stores each subdomain's own field, so the
+    global field equals the same subdomains generated as an ensemble."""
+    from scarl_amd import generator
+    gdim = (1024, 1024, 256)
+    starts, exts = generator.seeded_tiling(gdim, (8, 8, 2), 0, (112, 128, 144), 2024)
+    base = dict(model=3, corrL=(24.0, 24.0, 24.0), seed=31, margi=2, avg=1.0, std_dev=0.5)
+    _say(f"blend without overlap: extents {exts}")
+    with generator.Batch.blend(gdim, starts, exts, generator.FieldSpec(n=(2, 2, 2), **base), 32) as b:
+        b.run(use_graph=True)
+        b.sync()
+        G = b.output(-1)
+    specs, st = [], []
+    for k in range(2):
+        for j in range(8):
+            for i in range(8):
+                specs.append(generator.FieldSpec(n=(exts[0][i], exts[1][j], exts[2][k]), stream=len(specs), **base))
+                st.append((starts[0][i], starts[1][j], starts[2][k]))
+    with generator.Batch.ensemble(specs, 32) as e:
+        e.run()
+        e.sync()
+        worst = 0.0
+        for f, (s, spec) in enumerate(zip(st, specs)):
+            n = spec.n
+            sub = G[s[0]:s[0] + n[0], s[1]:s[1] + n[1], s[2]:s[2] + n[2]]
+            worst = max(worst, rel_l2(sub, e.output(f)))
+    _say(f"  worst subdomain rel L2 {worst:.2e}")
+    assert worst <= 1e-6
 
 
 def test_config4_blend_1024x1024x256(hip):
