@@ -53,6 +53,8 @@ struct FieldH {
     int nyq;               // 1: the kx = n1/2 column is a separate (n2, n3) plane after the rows
     int pad0;
     const void* W;         // host-exported noise (device copy), complex, dense (h1, n2, n3) layout
+    const void* kx0;       // 3D fields: a(k) W(k) of the kx = 0 plane, complex, dense (n2, n3), made
+                           // by k_h_kx0 before the z pass (device noise), or null
     void* ws;              // complex workspace: rows of wp elements (planes wz apart), then the dense
                            // (n2, n3) Nyquist plane (nyq)
     void* out;             // output base (real)
@@ -87,23 +89,13 @@ struct FieldR {
     double* out;           // (n1, n2, n3)
 };
 
-// Pencils of a strided pass (q = ky for the z pass, kz for the y pass), by tile kind:
-//   TILE_ROWS   : P = kx + wp q (kx < wp) over the workspace rows, P = pencil0 + p;
-//   TILE_NYQ    : q of the separate kx = n1/2 plane, q = pencil0 + p;
-//   TILE_KX0    : the kx = 0 pencils, P = wp q with q = pencil0 + p;
-//   TILE_OFFKX0 : the pencils off kx = 0, Q = pencil0 + p, kx = 1 + Q mod (wp - 1), q = Q / (wp - 1).
-// npen > 0 caps the tile's pencils below the plan's T.  The y pass takes TILE_ROWS and TILE_NYQ
-// tiles.  The generating z pass takes TILE_NYQ tiles, and per field either TILE_ROWS tiles (rows
-// of 8 or more aligned tiles: one tile in 8 or fewer holds a kx = 0 pencil) or TILE_KX0 tiles plus
-// tiles whose every pencil is off kx = 0: TILE_ROWS tiles aligned to T in kx with the first of a
-// row starting at kx = 1 (rows of a multiple of T, so no 128-byte segment spans two tiles), else
-// TILE_OFFKX0 tiles.
-enum TileKind { TILE_ROWS = 0, TILE_NYQ = 1, TILE_KX0 = 2, TILE_OFFKX0 = 3 };
+// Pencils of a strided pass: regular tiles cover P = kx + wp q (kx < wp) over the workspace rows,
+// Nyquist tiles cover q of the separate kx = n1/2 plane (q = ky for the z pass, kz for the y pass).
 struct Tile {
     int field;
     int pencil0;
-    int kind;
-    int npen;
+    int nyq;
+    int pad;
 };
 
 // Compile-time planned pass kernels: X(id, prec, layout, N, R0, R1, TPP); id 0 = generic runtime

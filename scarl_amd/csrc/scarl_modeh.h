@@ -92,11 +92,8 @@ __global__ __launch_bounds__(kBlock) void k_h_strided(const FieldH* __restrict__
     const PlanDev& pl = plans[axis == 2 ? f.plan_z : f.plan_y];
     const int wp = f.wp, n2 = f.n[1], n3 = f.n[2];
     const int nother = (axis == 2) ? n2 : n3;
-    const bool nyq = tl.kind == TILE_NYQ;
-    const int npen = min(tl.npen > 0 ? tl.npen : pl.T, (tl.kind == TILE_ROWS     ? wp * nother
-                                                        : tl.kind == TILE_OFFKX0 ? (wp - 1) * nother
-                                                                                 : nother) -
-                                                           tl.pencil0);
+    const bool nyq = tl.nyq != 0;
+    const int npen = min(pl.T, (nyq ? nother : wp * nother) - tl.pencil0);
     const int64_t wz = f.wz;
     const int64_t S = nyq ? ((axis == 2) ? (int64_t)n2 : 1) : ((axis == 2) ? wz : (int64_t)wp);
     cplx<T>* ws = reinterpret_cast<cplx<T>*>(f.ws) + (nyq ? wz * n3 : 0);
@@ -108,18 +105,8 @@ __global__ __launch_bounds__(kBlock) void k_h_strided(const FieldH* __restrict__
     auto prep = [&](int p) -> Ctx {
         const int P = tl.pencil0 + p;
         if (nyq) return {(axis == 2) ? (int64_t)P : (int64_t)n2 * P, wp, P};
-        int kx, q;
-        if (tl.kind == TILE_KX0) {
-            kx = 0;
-            q = P;
-        } else if (tl.kind == TILE_OFFKX0) {
-            kx = 1 + P % (wp - 1);
-            q = P / (wp - 1);
-        } else {
-            kx = P % wp;
-            q = P / wp;
-        }
-        const int64_t base = (axis == 2) ? kx + (int64_t)wp * q : kx + wz * q;
+        const int kx = P % wp, q = P / wp;
+        const int64_t base = (axis == 2) ? (int64_t)P : kx + wz * q;
         return {base, kx, q};
     };
     auto gld = [&](const Ctx& c, int n) -> cplx<T> {
@@ -173,6 +160,13 @@ __global__ __launch_bounds__(kBlock) void k_h_strided(const FieldH* __restrict__
                 }
             }
         }
+        // a kx = 0 pencil (self-conjugate: the draw above pairs the wrong points) takes the inputs
+        // k_h_kx0 made; every lane draws, so the generator stays out of any branch
+        if (f.kx0 != nullptr && c.kx == 0) {
+            const cplx<T>* z0 = reinterpret_cast<const cplx<T>*>(f.kx0) + c.k2;
+            lo = gload(z0 + (int64_t)f.n[1] * q);
+            if (has_hi) hi = gload(z0 + (int64_t)f.n[1] * (q + h3));
+        }
     };
     auto gpair = [&](const Ctx& c, int q, bool has_hi, cplx<T>& lo, cplx<T>& hi) {
         const int h3 = (pl.N + 1) >> 1;
@@ -187,15 +181,15 @@ __global__ __launch_bounds__(kBlock) void k_h_strided(const FieldH* __restrict__
     };
     if constexpr (GEN) {
         if (axis == 2) {
-            // Tiles whose pencils are all off the self-conjugate planes (TILE_OFFKX0, or a row
-            // range without kx = 0) with device noise: the first stage's butterflies hold the pairs
-            // (q, q + N/2), generated straight into registers. The test is uniform over the tile,
-            // so only the paired draw is unrolled into the butterflies. Other tiles (kx = 0, the
-            // Nyquist plane, host noise) stage the generated pencils through LDS.
+            // Row tiles with device noise: the first stage's butterflies hold the pairs
+            // (q, q + N/2), generated straight into registers. A kx = 0 pencil (self-conjugate)
+            // takes its inputs from the plane k_h_kx0 generated before this pass (or, without it,
+            // the tile stages through LDS). The test is uniform over the tile, so only the paired
+            // draw is unrolled into the butterflies. Nyquist tiles and host noise stage the
+            // generated pencils through LDS.
             const int kx_first = tl.pencil0 % wp;
-            const bool off_sc = tl.kind == TILE_OFFKX0 ||
-                                (tl.kind == TILE_ROWS && kx_first != 0 && kx_first + npen <= wp);
-            const bool direct = gen && f.src_mode == SRC_DEVICE_NOISE && off_sc;
+            const bool off_sc = f.kx0 != nullptr || (kx_first != 0 && kx_first + npen <= wp);
+            const bool direct = gen && !nyq && f.src_mode == SRC_DEVICE_NOISE && off_sc;
             if constexpr (Fft::kDirectGen) {
                 if (direct) {
                     Fft::template run_gen<T>(pl, pool, lds, npen, prep, gst, GenPair<decltype(gpair_dev)>{gpair_dev});
@@ -221,6 +215,20 @@ __global__ __launch_bounds__(kBlock) void k_h_strided(const FieldH* __restrict__
         }
     }
     Fft::template run<T, LAYOUT_PF, GEN, false>(pl, pool, lds, npen, prep, gld, gst);
+}
+
+// The kx = 0 plane's inputs a(k) W(k) of every 3D field with device noise (self-conjugate: the
+// draw of (0, ky, kz) is that of (0, -ky, -kz) conjugated), dense (n2, n3), for the z pass's
+// direct generation.  Block (x, y): points [256 x, 256 x + 256) of field y.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_h_kx0(const FieldH* __restrict__ fields) {
+    const FieldH& f = fields[blockIdx.y];
+    if (f.kx0 == nullptr || f.src_mode != SRC_DEVICE_NOISE) return;
+    const int n2 = f.n[1], n3 = f.n[2];
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= (int64_t)n2 * n3) return;
+    const int ky = (int)(i % n2), kz = (int)(i / n2);
+    gstore(reinterpret_cast<cplx<T>*>(const_cast<void*>(f.kx0)) + i, generate_h<T>(f, 0, ky, kz));
 }
 
 // x pass: for even n1 a half-length complex FFT of
@@ -509,6 +517,12 @@ struct ModeHDispatch {
                                       const void* pool, size_t lds, hipStream_t s) {                       \
         return ModeHDispatch<T, PREC, E1, E2>::run_strided(axis, gen, spec, ecap, fields, tiles, tile0, ntiles, \
                                                            plans, pool, lds, s);                           \
+    }                                                                                                      \
+    hipError_t launch_h_kx0_##SFX(const FieldH* fields, int nfields, int64_t max_pts, hipStream_t s) {     \
+        if (nfields <= 0 || max_pts <= 0) return hipSuccess;                                               \
+        hipLaunchKernelGGL((k_h_kx0<T>), dim3((unsigned)((max_pts + kBlock - 1) / kBlock), (unsigned)nfields), \
+                           dim3(kBlock), 0, s, fields);                                                     \
+        return hipGetLastError();                                                                          \
     }                                                                                                      \
     hipError_t launch_h_xpass_##SFX(int spec, int ecap, bool blend, bool full, const FieldH* fields,       \
                                     const Tile* tiles, int tile0, int ntiles, const PlanDev* plans,        \

@@ -30,6 +30,8 @@ hipError_t launch_h_strided_f32(int axis, bool gen, int spec, int ecap, const Fi
 hipError_t launch_h_strided_f64(int axis, bool gen, int spec, int ecap, const FieldH* fields, const Tile* tiles,
                                 int tile0, int ntiles, const PlanDev* plans, const void* pool, size_t lds,
                                 hipStream_t s);
+hipError_t launch_h_kx0_f32(const FieldH* fields, int nfields, int64_t max_pts, hipStream_t s);
+hipError_t launch_h_kx0_f64(const FieldH* fields, int nfields, int64_t max_pts, hipStream_t s);
 hipError_t launch_h_xpass_f32(int spec, int ecap, bool blend, bool full, const FieldH* fields, const Tile* tiles, int tile0,
                               int ntiles, const PlanDev* plans, const void* pool, MomPartial* partials, size_t lds,
                               hipStream_t s);
@@ -853,6 +855,8 @@ struct scarl_batch {
     int nslots = 0;
     void* d_ws = nullptr;       // all half-spectrum workspaces
     void* d_out = nullptr;      // all outputs (or the global field)
+    void* d_kx0 = nullptr;      // the kx = 0 planes' inputs of the 3D fields (k_h_kx0)
+    int64_t kx0_max = 0;        // largest n2 n3 among them (0: no 3D field)
     int64_t bytes = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -865,7 +869,7 @@ struct scarl_batch {
         for (auto& f : fields)
             if (f.noise_buf) hipFree(f.noise_buf);
         void* ptrs[] = {d_fields, d_tz, d_ty, d_tx, d_field_xtiles, d_partials, d_field_mom, d_slot_count, d_slot_out,
-                        d_ws, d_out};
+                        d_ws, d_out, d_kx0};
         for (void* p : ptrs)
             if (p) hipFree(p);
     }
@@ -1002,29 +1006,15 @@ static int finish_amplitudes(scarl_batch* b) {
 // regular tiles that hold a kx = 0 pencil (kind 2) and the other regular tiles of the workspace
 // rows (kind 0).  The first two carry the self-conjugate noise, whose unpaired draws make them
 // costlier in the generating pass.
-static void strided_tiles(int field, const FieldH& f, int nother, int T, int sel, std::vector<Tile>& out) {
-    if (sel == 1) {
+static void strided_tiles(int field, const FieldH& f, int nother, int T, int kind, std::vector<Tile>& out) {
+    if (kind == 1) {
         if (f.nyq)
-            for (int p = 0; p < nother; p += T) out.push_back({field, p, TILE_NYQ, 0});
-        return;
-    }
-    if (sel == 3) {
-        for (int p = 0; p < nother; p += T) out.push_back({field, p, TILE_KX0, 0});
-        return;
-    }
-    if (sel == 4) {
-        if (T >= 2 && f.wp % T == 0) {
-            // rows of a multiple of T: tiles aligned to T in kx, the first of a row from kx = 1
-            for (int64_t p = 0; p < (int64_t)f.wp * nother; p += T)
-                out.push_back(p % f.wp == 0 ? Tile{field, (int)p + 1, TILE_ROWS, T - 1} : Tile{field, (int)p, TILE_ROWS, 0});
-        } else {
-            for (int64_t p = 0; p < (int64_t)(f.wp - 1) * nother; p += T) out.push_back({field, (int)p, TILE_OFFKX0, 0});
-        }
+            for (int p = 0; p < nother; p += T) out.push_back({field, p, 1, 0});
         return;
     }
     for (int64_t p = 0; p < (int64_t)f.wp * nother; p += T) {
         const bool has_kx0 = (p % f.wp == 0) || (p / f.wp != (p + std::min<int64_t>(T, (int64_t)f.wp * nother - p) - 1) / f.wp);
-        if (has_kx0 == (sel == 2)) out.push_back({field, (int)p, TILE_ROWS, 0});
+        if (has_kx0 == (kind == 2)) out.push_back({field, (int)p, 0, 0});
     }
 }
 
@@ -1075,21 +1065,9 @@ static void make_tiles(scarl_batch* b) {
             yp[f.plan_y].push_back(i);
         }
         for (auto& kv : zp) {
-            // The generating z pass, Nyquist tiles first, then the tiles holding a kx = 0 pencil
-            // (which stage their noise through LDS), then the rest (which draw it straight into
-            // the first stage).  Rows of at least 8 aligned tiles keep the row tiles (one slow
-            // tile in 8 or fewer; measured on 512^3 fields, the kx = 0 column tiles with their
-            // split 128-byte segments cost the y and x passes more than they save); shorter or
-            // unaligned rows take kx = 0 column tiles and off-kx = 0 tiles, so that nearly every
-            // tile draws directly (config 5's 64^3 rows are one tile each).
             const int t0 = (int)b->tz.size(), T = b->ps.plans[kv.first].T;
-            auto rows = [&](const FieldH& f) { return T >= 2 && f.wp % T == 0 && f.wp >= 8 * T; };
-            for (int phase = 0; phase < 3; ++phase)
-                for (int i : kv.second) {
-                    const FieldH& f = b->fields[i].f;
-                    const int sel = phase == 0 ? 1 : rows(f) ? (phase == 1 ? 2 : 0) : (phase == 1 ? 3 : 4);
-                    strided_tiles(i, f, f.n[1], T, sel, b->tz);
-                }
+            for (int kind : {1, 2, 0})
+                for (int i : kv.second) strided_tiles(i, b->fields[i].f, b->fields[i].f.n[1], T, kind, b->tz);
             g.cz.push_back({kv.first, t0, (int)b->tz.size()});
         }
         for (auto& kv : yp) {
@@ -1178,6 +1156,22 @@ static int finalize_batch(scarl_batch* b, const std::vector<double>& slot_count)
         out_elems = (int64_t)b->gdim[0] * b->gdim[1] * b->gdim[2];
     SCARL_TRY(dmalloc(&b->d_ws, (size_t)ws_elems * 2 * rsz));
     SCARL_TRY(dmalloc(&b->d_out, (size_t)out_elems * rsz));
+    int64_t kx0_elems = 0;
+    b->kx0_max = 0;
+    for (auto& fh : b->fields)
+        if (fh.f.n[2] > 1) {
+            kx0_elems += (int64_t)fh.f.n[1] * fh.f.n[2];
+            b->kx0_max = std::max<int64_t>(b->kx0_max, (int64_t)fh.f.n[1] * fh.f.n[2]);
+        }
+    if (kx0_elems > 0) {
+        SCARL_TRY(dmalloc(&b->d_kx0, (size_t)kx0_elems * 2 * rsz));
+        int64_t ko = 0;
+        for (auto& fh : b->fields)
+            if (fh.f.n[2] > 1) {
+                fh.f.kx0 = static_cast<char*>(b->d_kx0) + ko * 2 * rsz;
+                ko += (int64_t)fh.f.n[1] * fh.f.n[2];
+            }
+    }
     b->bytes = ws_elems * 2 * rsz + out_elems * rsz;
     int64_t wo = 0, oo = 0;
     for (auto& fh : b->fields) {
@@ -1253,6 +1247,15 @@ static int enqueue_batch(scarl_batch* b) {
     if (pipe) {
         HIP_TRY(hipEventRecord(b->gev[0], s));
         HIP_TRY(hipStreamWaitEvent(g_ctx.gen_stream, b->gev[0], 0));
+    }
+    if (b->kx0_max > 0) {
+        // the kx = 0 planes first, on the stream of the generating passes
+        SCARL_TRY(timed("h_kx0", [&] {
+            const int nf = (int)b->fields.size();
+            hipStream_t zs = pipe ? g_ctx.gen_stream : s;
+            return b->fp64 ? launch_h_kx0_f64(b->d_fields, nf, b->kx0_max, zs)
+                           : launch_h_kx0_f32(b->d_fields, nf, b->kx0_max, zs);
+        }));
     }
     for (size_t g = 0; g < b->groups.size(); ++g)
         SCARL_TRY(enqueue_group(b, b->groups[g], pipe ? g_ctx.gen_stream : s, s, pipe ? b->gev[g + 1] : nullptr));
