@@ -98,24 +98,28 @@ __global__ __launch_bounds__(kBlock) void k_h_strided(const FieldH* __restrict__
     const int64_t S = nyq ? ((axis == 2) ? (int64_t)n2 : 1) : ((axis == 2) ? wz : (int64_t)wp);
     cplx<T>* ws = reinterpret_cast<cplx<T>*>(f.ws) + (nyq ? wz * n3 : 0);
     const bool gen = GEN && (f.first_axis == axis) && (f.src_mode != SRC_WORKSPACE);
+    // Element n of a pencil sits n * Sb bytes past the pencil's first element: one 32 x 32 -> 64-bit
+    // multiply-add per access (the planner keeps a plane pitch below 4 GiB).
+    const uint32_t Sb = (uint32_t)(S * (int64_t)sizeof(cplx<T>));
     struct Ctx {
-        int64_t base;
+        char* row;   // the pencil's first element
         int kx, k2;  // k2 = ky (z pass) or kz (y pass)
     };
     auto prep = [&](int p) -> Ctx {
         const int P = tl.pencil0 + p;
-        if (nyq) return {(axis == 2) ? (int64_t)P : (int64_t)n2 * P, wp, P};
+        if (nyq) return {reinterpret_cast<char*>(ws + ((axis == 2) ? (int64_t)P : (int64_t)n2 * P)), wp, P};
         const int kx = P % wp, q = P / wp;
         const int64_t base = (axis == 2) ? (int64_t)P : kx + wz * q;
-        return {base, kx, q};
+        return {reinterpret_cast<char*>(ws + base), kx, q};
     };
+    auto at = [&](const Ctx& c, int n) { return reinterpret_cast<cplx<T>*>(c.row + (uint64_t)(uint32_t)n * Sb); };
     auto gld = [&](const Ctx& c, int n) -> cplx<T> {
         if constexpr (GEN) {
             if (gen) return (axis == 2) ? generate_h<T>(f, c.kx, c.k2, n) : generate_h<T>(f, c.kx, n, c.k2);
         }
-        return gload(ws + (c.base + n * S));
+        return gload(at(c, n));
     };
-    auto gst = [&](const Ctx& c, int n, cplx<T> v) { gstore(ws + (c.base + n * S), v); };
+    auto gst = [&](const Ctx& c, int n, cplx<T> v) { gstore(at(c, n), v); };
     // Generation along z: the points q and q + h3 (h3 = ceil(n3/2)) of a pencil share one Philox
     // call (noise_h). gpair_dev produces both for device noise off the self-conjugate planes;
     // gpair falls back to one gld per point on those planes, for host-exported noise and for a

@@ -1146,6 +1146,9 @@ static int finalize_batch(scarl_batch* b, const std::vector<double>& slot_count)
         FieldH& f = fh.f;
         const int64_t plane = (int64_t)f.wp * f.n[1];
         f.wz = plane + ((f.n[2] > 1 && (plane * 2 * rsz) % 65536 == 0) ? 4096 / (2 * (int64_t)rsz) : 0);
+        // the strided passes step along a pencil by a 32-bit byte pitch
+        if (f.wz * 2 * (int64_t)rsz >= (int64_t(1) << 32))
+            return fail(SCARL_E_UNSUPPORTED, "a kz plane of %lld elements exceeds 4 GiB", (long long)f.wz);
         ws_elems += f.wz * f.n[2] + (int64_t)f.nyq * f.n[1] * f.n[2];
     }
     b->any_2d = false;
