@@ -259,7 +259,10 @@ __global__ __launch_bounds__(kBlock) void k_h_xpass(const FieldH* __restrict__ f
     const int n1 = f.n[0], n2 = f.n[1], n3 = f.n[2], M = n1 / 2, h1 = f.h1;
     constexpr bool full = FULL;  // odd n1 (full-length C2C over the Hermitian extension)
     const int64_t nrows = (int64_t)n2 * n3;
-    const int npen = (int)min((int64_t)pl.T, nrows - (int64_t)tl.pencil0);
+    // odd n1: a pencil is a pair of rows (2P, 2P + 1), transformed together as X_a + i X_b (both
+    // outputs are real: row 2P is the real part, row 2P + 1 the imaginary part)
+    const int64_t npencils = full ? (nrows + 1) / 2 : nrows;
+    const int npen = (int)min((int64_t)pl.T, npencils - (int64_t)tl.pencil0);
     const cplx<T>* ws = reinterpret_cast<const cplx<T>*>(f.ws);
     const cplx<T>* pre = pool + pl.pre_off;
     T* out = reinterpret_cast<T*>(f.out);
@@ -273,6 +276,11 @@ __global__ __launch_bounds__(kBlock) void k_h_xpass(const FieldH* __restrict__ f
         T* o;
         T swyz;    // square root of the row's PoU weight (blend)
         int myz;   // overlap mask bits of the row (blend): 2 y, 4 z
+        // odd n1: the pair's second row (X2 null past the last row)
+        const cplx<T>* X2;
+        T* o2;
+        T swyz2;
+        int myz2;
     };
     // blend: the x factors of the PoU weight's square root for the tile's rows (all of one field),
     // in LDS after the transform's tile (1 off the ramps)
@@ -286,27 +294,59 @@ __global__ __launch_bounds__(kBlock) void k_h_xpass(const FieldH* __restrict__ f
         return (in_overlap(ky, n2, f.nov_lo[1], f.nov_hi[1]) ? 2 : 0) |
                (in_overlap(kz, n3, f.nov_lo[2], f.nov_hi[2]) ? 4 : 0);
     };
-    auto prep = [&](int p) -> Ctx {
-        const int row = tl.pencil0 + p;
+    auto row_ctx = [&](int row, const cplx<T>*& X, T*& o, T& swyz, int& myz) {
         const int ky = row % n2, kz = row / n2;
-        Ctx c = {ws + (int64_t)ky * f.wp + kz * f.wz, ws + f.wz * n3 + row, out + ky * f.out_ld1 + kz * f.out_ld2,
-                 T(1), 0};
+        X = ws + (int64_t)ky * f.wp + kz * f.wz;
+        o = out + ky * f.out_ld1 + kz * f.out_ld2;
+        swyz = T(1);
+        myz = 0;
         if (BLEND) {
-            c.swyz = pou_sqrt_factor<T>(ky, n2, f.nov_lo[1], f.nov_hi[1]) *
-                     pou_sqrt_factor<T>(kz, n3, f.nov_lo[2], f.nov_hi[2]);
-            c.myz = row_mask(row);
+            swyz = pou_sqrt_factor<T>(ky, n2, f.nov_lo[1], f.nov_hi[1]) *
+                   pou_sqrt_factor<T>(kz, n3, f.nov_lo[2], f.nov_hi[2]);
+            myz = row_mask(row);
         }
+    };
+    auto prep = [&](int p) -> Ctx {
+        Ctx c;
+        if (full) {
+            const int64_t row = 2 * ((int64_t)tl.pencil0 + p);
+            row_ctx((int)row, c.X, c.o, c.swyz, c.myz);
+            c.Xn = nullptr;
+            if (row + 1 < nrows) {
+                row_ctx((int)row + 1, c.X2, c.o2, c.swyz2, c.myz2);
+            } else {
+                c.X2 = nullptr;
+                c.o2 = nullptr;
+                c.swyz2 = T(1);
+                c.myz2 = 0;
+            }
+            return c;
+        }
+        const int row = tl.pencil0 + p;
+        row_ctx(row, c.X, c.o, c.swyz, c.myz);
+        c.Xn = ws + f.wz * n3 + row;
+        c.X2 = nullptr;
+        c.o2 = nullptr;
+        c.swyz2 = T(1);
+        c.myz2 = 0;
         return c;
+    };
+    // element k of the Hermitian extension of a row (X_k below h1, conj X_{n1-k} above; the
+    // imaginary part of X_0 ignored)
+    auto hext = [&](const cplx<T>* X, int k) -> cplx<T> {
+        if (k < h1) {
+            cplx<T> a = gload(X + k);
+            if (k == 0) a.y = T(0);
+            return a;
+        }
+        const cplx<T> a = gload(X + (n1 - k));
+        return {a.x, -a.y};
     };
     auto gld = [&](const Ctx& c, int k) -> cplx<T> {
         if (full) {
-            if (k < h1) {
-                cplx<T> a = gload(c.X + k);
-                if (k == 0) a.y = T(0);
-                return a;
-            }
-            const cplx<T> a = gload(c.X + (n1 - k));
-            return {a.x, -a.y};
+            const cplx<T> a = hext(c.X, k);
+            const cplx<T> bb = c.X2 ? hext(c.X2, k) : cplx<T>{T(0), T(0)};
+            return {a.x - bb.y, a.y + bb.x};
         }
         cplx<T> a = gload(c.X + k), bb = gload(k ? c.X + (M - k) : c.Xn);
         if (k == 0) {
@@ -329,6 +369,15 @@ __global__ __launch_bounds__(kBlock) void k_h_xpass(const FieldH* __restrict__ f
                     ms2 += d * d;
                 }
                 gstore1(o + k, r);
+                if (c.o2) {
+                    const T r2 = apply_margi<T>(z.y, f.margi, mu, sd);
+                    if (mom) {
+                        const double d = r2;
+                        ms += d;
+                        ms2 += d * d;
+                    }
+                    gstore1(c.o2 + k, r2);
+                }
                 return;
             }
             const T r0 = apply_margi<T>(z.x, f.margi, mu, sd), r1 = apply_margi<T>(z.y, f.margi, mu, sd);
@@ -346,15 +395,18 @@ __global__ __launch_bounds__(kBlock) void k_h_xpass(const FieldH* __restrict__ f
         // if it is not its first owner, and the marginal and the moments count only at its last
         // owner (a zero contribution elsewhere).  Even n1: both outputs of z go out as one
         // 2-element store when the row is aligned for it.
-        const int ne = full ? 1 : 2;
+        // odd n1: element k of the pair's two rows (the second may be absent)
+        const int ne = full ? (c.o2 ? 2 : 1) : 2;
         T val[2];
 #pragma unroll
         for (int e = 0; e < 2; ++e) {
             if (e >= ne) break;
             const int i = full ? k : 2 * k + e;
-            const int msk = c.myz | (in_overlap(i, n1, f.nov_lo[0], f.nov_hi[0]) ? 1 : 0);
-            const T v0 = (e ? z.y : z.x) * (c.swyz * wxl[i]);
-            const T v = (f.color & msk) ? v0 + gload1(o + i) : v0;  // not the first owner: add
+            const bool r2 = full && e == 1;
+            T* oe = r2 ? c.o2 : o;
+            const int msk = (r2 ? c.myz2 : c.myz) | (in_overlap(i, n1, f.nov_lo[0], f.nov_hi[0]) ? 1 : 0);
+            const T v0 = (e ? z.y : z.x) * ((r2 ? c.swyz2 : c.swyz) * wxl[i]);
+            const T v = (f.color & msk) ? v0 + gload1(oe + i) : v0;  // not the first owner: add
             const bool last = (f.color & msk) == msk;
             const T r = apply_margi<T>(v, f.margi, mu, sd);
             val[e] = last ? r : v;
@@ -364,10 +416,13 @@ __global__ __launch_bounds__(kBlock) void k_h_xpass(const FieldH* __restrict__ f
                 ms2 += d * d;
             }
         }
-        if (!full && (reinterpret_cast<uintptr_t>(o) % (2 * sizeof(T))) == 0) {
+        if (full) {
+            gstore1(o + k, val[0]);
+            if (ne == 2) gstore1(c.o2 + k, val[1]);
+        } else if ((reinterpret_cast<uintptr_t>(o) % (2 * sizeof(T))) == 0) {
             gstore(reinterpret_cast<cplx<T>*>(o + 2 * k), cplx<T>{val[0], val[1]});
         } else {
-            for (int e = 0; e < ne; ++e) gstore1(o + (full ? k : 2 * k + e), val[e]);
+            for (int e = 0; e < 2; ++e) gstore1(o + 2 * k + e, val[e]);
         }
         }
     };

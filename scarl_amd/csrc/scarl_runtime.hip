@@ -1084,7 +1084,8 @@ static void make_tiles(scarl_batch* b) {
                 const int t0 = (int)b->tx.size(), T = b->ps.plans[kv.first].T;
                 for (int i : kv.second) {
                     const FieldH& f = b->fields[i].f;
-                    const int64_t rows = (int64_t)f.n[1] * f.n[2];
+                    // odd n1: a pencil of the x pass is a pair of rows (one complex transform)
+                    const int64_t rows = (f.n[0] & 1) ? ((int64_t)f.n[1] * f.n[2] + 1) / 2 : (int64_t)f.n[1] * f.n[2];
                     b->field_xtiles[i].x = (int)b->tx.size();
                     // last rows first: the y pass wrote the high kz planes last, so part of them is
                     // still in the 256 MB Infinity Cache when the x pass starts (1.3 % on the x pass)
