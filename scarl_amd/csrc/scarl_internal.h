@@ -53,7 +53,7 @@ struct FieldH {
     int nyq;               // 1: the kx = n1/2 column is a separate (n2, n3) plane after the rows
     int pad0;
     const void* W;         // host-exported noise (device copy), complex, dense (h1, n2, n3) layout
-    const void* kx0;       // 3D fields: a(k) W(k) of the kx = 0 plane, complex, dense (n2, n3), made
+    const void* kx0;       // 3D fields: a(k) W(k) of the kx = 0 plane, complex, dense (n3, n2), made
                            // by k_h_kx0 before the z pass (device noise), or null
     void* ws;              // complex workspace: rows of wp elements (planes wz apart), then the dense
                            // (n2, n3) Nyquist plane (nyq)

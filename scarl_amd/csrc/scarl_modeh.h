@@ -167,9 +167,9 @@ __global__ __launch_bounds__(kBlock) void k_h_strided(const FieldH* __restrict__
         // a kx = 0 pencil (self-conjugate: the draw above pairs the wrong points) takes the inputs
         // k_h_kx0 made; every lane draws, so the generator stays out of any branch
         if (f.kx0 != nullptr && c.kx == 0) {
-            const cplx<T>* z0 = reinterpret_cast<const cplx<T>*>(f.kx0) + c.k2;
-            lo = gload(z0 + (int64_t)f.n[1] * q);
-            if (has_hi) hi = gload(z0 + (int64_t)f.n[1] * (q + h3));
+            const cplx<T>* z0 = reinterpret_cast<const cplx<T>*>(f.kx0) + (int64_t)f.n[2] * c.k2;
+            lo = gload(z0 + q);
+            if (has_hi) hi = gload(z0 + (q + h3));
         }
     };
     auto gpair = [&](const Ctx& c, int q, bool has_hi, cplx<T>& lo, cplx<T>& hi) {
@@ -222,8 +222,9 @@ __global__ __launch_bounds__(kBlock) void k_h_strided(const FieldH* __restrict__
 }
 
 // The kx = 0 plane's inputs a(k) W(k) of every 3D field with device noise (self-conjugate: the
-// draw of (0, ky, kz) is that of (0, -ky, -kz) conjugated), dense (n2, n3), for the z pass's
-// direct generation.  Block (x, y): points [256 x, 256 x + 256) of field y.
+// draw of (0, ky, kz) is that of (0, -ky, -kz) conjugated), dense (n3, n2) with kz fastest (a z
+// pencil's inputs are contiguous), for the z pass's direct generation.  Block (x, y): points
+// [256 x, 256 x + 256) of field y.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_h_kx0(const FieldH* __restrict__ fields) {
     const FieldH& f = fields[blockIdx.y];
@@ -231,7 +232,7 @@ __global__ __launch_bounds__(kBlock) void k_h_kx0(const FieldH* __restrict__ fie
     const int n2 = f.n[1], n3 = f.n[2];
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i >= (int64_t)n2 * n3) return;
-    const int ky = (int)(i % n2), kz = (int)(i / n2);
+    const int kz = (int)(i % n3), ky = (int)(i / n3);
     gstore(reinterpret_cast<cplx<T>*>(const_cast<void*>(f.kx0)) + i, generate_h<T>(f, 0, ky, kz));
 }
 

@@ -680,6 +680,15 @@ def test_config4_blend_1024x1024x256(hip):
     with generator.Batch.ensemble(specs, 32) as e:
         e.run()
         fields = [e.output(i).astype(np.float64) for i in range(len(specs))]
+    # the subdomain fields themselves against the FP64 oracle (same device noise stream): the first
+    # and the last subdomain, and one of every x extent in between
+    for sid in sorted({0, len(specs) - 1} | {next(q for q, sp in enumerate(specs) if sp.n[0] == e0)
+                                              for e0 in set(exts[0])}):
+        sp = specs[sid]
+        ref_f = O.gen_mode_h(sp.n, (1, 1, 1), 3, (24.0, 24.0, 24.0), seed=31, stream=sid, nthreads=16)
+        err_f = rel_l2(fields[sid], ref_f)
+        _say(f"config 4: subdomain {sid} {sp.n} vs FP64 oracle rel L2 {err_f:.2e}")
+        assert err_f <= FP32_TOL
     ref = O.blend_global(fields, st, gdim, ws)
     err = rel_l2(G, ref)
     _say(f"config 4: blend vs gather-sum rel L2 {err:.2e}")
