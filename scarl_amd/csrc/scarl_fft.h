@@ -446,6 +446,32 @@ __device__ __forceinline__ void nadd(double& s, double& c, double v) {
     s = t;
 }
 
+// Block reduction of plain per-thread sums (s, s2) of one tile: plain FP64 sums within each
+// 64-lane wave (at most 64 partials, each of a few dozen values), then the four wave sums combined
+// with compensation by one thread; red: 8 doubles of LDS.  One barrier, where block_reduce_mom
+// takes eight.
+__device__ __forceinline__ void block_reduce_sums(double s, double s2, double* red, MomPartial* out) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        s += __shfl_xor(s, off);
+        s2 += __shfl_xor(s2, off);
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) {
+        red[w] = s;
+        red[4 + w] = s2;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = red[0], ca = 0.0, b = red[4], cb = 0.0;
+        for (int i = 1; i < kBlock / 64; ++i) {
+            nadd(a, ca, red[i]);
+            nadd(b, cb, red[4 + i]);
+        }
+        *out = {a, ca, b, cb};
+    }
+}
+
 // Block reduction of compensated pairs; red: 4 * kBlock doubles of LDS.
 __device__ __forceinline__ void block_reduce_mom(double s, double c, double s2, double c2, double* red,
                                                  MomPartial* out) {

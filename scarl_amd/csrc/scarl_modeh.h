@@ -268,7 +268,7 @@ __global__ __launch_bounds__(kBlock) void k_h_xpass(const FieldH* __restrict__ f
     const cplx<T>* pre = pool + pl.pre_off;
     T* out = reinterpret_cast<T*>(f.out);
     const T mu = T(f.mu), sd = T(f.sd);
-    double ms = 0.0, mc = 0.0, ms2 = 0.0, mc2 = 0.0;
+    double ms = 0.0, ms2 = 0.0;
     const bool mom = f.mom_slot >= 0;
 
     struct Ctx {
@@ -450,7 +450,7 @@ __global__ __launch_bounds__(kBlock) void k_h_xpass(const FieldH* __restrict__ f
     }
     if (mom) {
         __syncthreads();
-        block_reduce_mom(ms, mc, ms2, mc2, reinterpret_cast<double*>(smem), partials + tix);
+        block_reduce_sums(ms, ms2, reinterpret_cast<double*>(smem), partials + tix);
     }
 }
 
