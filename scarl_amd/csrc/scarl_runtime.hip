@@ -326,7 +326,8 @@ struct PlanChoice {
     int spec = 0;   // PV_FIXED: SCARL_FIXED_SPECS id
     int ecap = 0;   // PV_GENERIC: complex elements per thread and stage (16 or 32; 0 = the set's)
     int order = 0;  // PV_GENERIC: 0 = largest radix first, 1 = smallest first
-    int tdiv = 1;   // PV_GENERIC / PV_MFMA: pencils per tile divided by this
+    int tdiv = 1;   // PV_GENERIC / PV_MFMA: pencils per tile divided by this; PV_GENERIC 0: rounded
+                    // down to a multiple of 16 (whole 128-byte rows in a pencil-fastest tile)
     int skew = 1;   // PV_GENERIC / PV_MFMA, butterfly-fastest layout: one LDS element of skew per 16
     int comp = 0;   // PV_GENERIC: 1 = fewest stages with the composite radices 6, 9, 12 allowed
 };
@@ -355,7 +356,7 @@ static bool parse_choice(const char* txt, PlanChoice& c) {
     if (sscanf(txt, "fixed:%d", &c.spec) == 1) { c.variant = PV_FIXED; return true; }
     if (sscanf(txt, "generic:e%d:o%d:t%d:s%d:c%d", &c.ecap, &c.order, &c.tdiv, &c.skew, &c.comp) == 5) {
         c.variant = PV_GENERIC;
-        return (c.ecap == 16 || c.ecap == 32) && c.tdiv >= 1 && c.tdiv <= 8;
+        return (c.ecap == 16 || c.ecap == 32) && c.tdiv >= 0 && c.tdiv <= 8;
     }
     if (sscanf(txt, "mfma:t%d:s%d", &c.tdiv, &c.skew) == 2) {
         c.variant = PV_MFMA;
@@ -692,7 +693,10 @@ struct PlanSet {
                                   pl.spec == kSpecGenericComp;
         if (runtime_tile) pl.T = std::min(pl.T, Tmax);
         if (pl.spec == kSpecMfmaGeneric) pl.T = std::max(1, std::min(pl.T, kMfmaTileElems / N));
-        if (runtime_tile) pl.T = std::max(1, pl.T / tdiv);
+        // tdiv = 0: a tile of 16 or more pencils cut to a multiple of 16, so that every access of a
+        // pencil-fastest tile covers whole 128-byte rows (the autotuner picks it for config 4's
+        // 144-point y pass: 21 -> 16 pencils, 0.449 -> 0.355 ms); any other tdiv divides T
+        if (runtime_tile) pl.T = tdiv == 0 ? (pl.T >= 16 ? pl.T / 16 * 16 : pl.T) : std::max(1, pl.T / tdiv);
         pl.lds_elems = (int)(pl.T * (layout == LAYOUT_PF ? NL : pl.ldp));
         if (kind == 1) {
             pl.pre_off = (int)(pool.size() / 2);
@@ -1338,7 +1342,7 @@ static std::vector<PlanChoice> plan_candidates(const PlanSet& ps, int N, int lay
     for (int ec : ps.fp64 ? std::vector<int>{16} : std::vector<int>{16, 32})
         for (int comp : {0, 1})
             for (int order : {0, 1})
-                for (int tdiv : {1, 2})
+                for (int tdiv : layout == LAYOUT_PF ? std::vector<int>{1, 2, 0} : std::vector<int>{1, 2})
                     for (int sk : skews) {
                         PlanChoice g;
                         g.variant = PV_GENERIC;

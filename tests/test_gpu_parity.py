@@ -949,7 +949,10 @@ FORCED = [((192, 96, 144), {"fp32/144/0/0/1": "fixed:19", "fp32/96/0/0/0": "fixe
           ((144, 216, 80), {"fp32/80/0/0/1": "mfma:t2:s1", "fp32/216/0/0/0": "generic:e16:o0:t2:s1:c1",
                             "fp32/72/1/1/2": "fixed:21"}),
           ((144, 216, 72), {"fp64/72/0/0/1": "generic:e16:o1:t1:s1:c1", "fp64/216/0/0/0": "generic:e16:o0:t2:s1:c1",
-                            "fp64/72/1/1/2": "generic:e16:o1:t2:s0:c1"})]
+                            "fp64/72/1/1/2": "generic:e16:o1:t2:s0:c1"}),
+          # tiles cut to a multiple of 16 pencils (t0): 21 -> 16 for 144, 21 -> 16 for 192
+          ((160, 144, 192), {"fp32/192/0/0/1": "generic:e16:o0:t0:s1:c1", "fp32/144/0/0/0": "generic:e16:o0:t0:s1:c1",
+                             "fp32/80/1/1/2": "generic:e16:o0:t1:s1:c0"})]
 
 
 @pytest.mark.parametrize("case", range(len(FORCED)))
