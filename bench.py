@@ -281,6 +281,7 @@ def _pass_table(batch, nruns, step_bytes):
 def _plans(batch):
     return [f"{r['pass']} N={r['N']} {r['kernel']} {r['radices']} T={r['T']} {r['choice']}"
             + (f" tuned_ms={r['tuned_ms']} default_ms={r['default_ms']}" if "tuned_ms" in r else "")
+            + (f" step_ms={r['step_ms']} default_step_ms={r['default_step_ms']}" if "step_ms" in r else "")
             + (" cached" if r.get("cached") else "") for r in batch.plan_report()]
 
 
