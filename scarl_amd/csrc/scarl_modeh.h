@@ -101,16 +101,42 @@ __global__ __launch_bounds__(kBlock) void k_h_strided(const FieldH* __restrict__
     // Element n of a pencil sits n * Sb bytes past the pencil's first element: one 32 x 32 -> 64-bit
     // multiply-add per access (the planner keeps a plane pitch below 4 GiB).
     const uint32_t Sb = (uint32_t)(S * (int64_t)sizeof(cplx<T>));
+    // The field's FP32 generation constants, read once: per Philox call they would be reloaded
+    // (and the FP64 ones converted again), since the descriptors sit in global memory that the
+    // compiler cannot prove disjoint from the pass's stores.
+    const bool gauss = f.model == CM_GAUSSIAN;
+    const float g_a0 = float(f.alpha[0]), g_a1 = float(f.alpha[1]), g_az = float(f.alpha[2]);
+    const float g_e2 = gauss ? -0.36067376022224085f : float(2.0 * f.expo);
+    const float g_cc = float(-0.69314718055994530942 * f.amp0 * f.amp0);
+    const uint64_t g_seed = f.seed;
+    const uint32_t g_stream = f.stream;
+    const int g_h1 = f.h1, g_N = pl.N;
+    // Philox counters of a z pencil step by h1 n2 per kz (below 2^32: a kz plane of the workspace
+    // stays under 4 GiB)
+    const uint32_t g_cstride = (uint32_t)g_h1 * (uint32_t)n2;
+    const cplx<T>* g_kx0 = reinterpret_cast<const cplx<T>*>(f.kx0);
     struct Ctx {
-        char* row;   // the pencil's first element
-        int kx, k2;  // k2 = ky (z pass) or kz (y pass)
+        char* row;     // the pencil's first element
+        int kx, k2;    // k2 = ky (z pass) or kz (y pass)
+        float qxy;     // z pass, FP32 generation: (alpha_x kx)^2 + (alpha_y f_y)^2 (+1 Matern)
+        uint64_t ctr;  // z pass: Philox counter of the pencil's kz = 0 point
     };
     auto prep = [&](int p) -> Ctx {
         const int P = tl.pencil0 + p;
-        if (nyq) return {reinterpret_cast<char*>(ws + ((axis == 2) ? (int64_t)P : (int64_t)n2 * P)), wp, P};
-        const int kx = P % wp, q = P / wp;
-        const int64_t base = (axis == 2) ? (int64_t)P : kx + wz * q;
-        return {reinterpret_cast<char*>(ws + base), kx, q};
+        Ctx c;
+        if (nyq) {
+            c = {reinterpret_cast<char*>(ws + ((axis == 2) ? (int64_t)P : (int64_t)n2 * P)), wp, P, 0.0f, 0};
+        } else {
+            const int kx = P % wp, q = P / wp;
+            const int64_t base = (axis == 2) ? (int64_t)P : kx + wz * q;
+            c = {reinterpret_cast<char*>(ws + base), kx, q, 0.0f, 0};
+        }
+        if constexpr (GEN && sizeof(T) == 4) {
+            const float ax = g_a0 * float(c.kx), ay = g_a1 * float(signed_freq(c.k2, n2));
+            c.qxy = fmaf(ax, ax, ay * ay) + (gauss ? 0.0f : 1.0f);
+            c.ctr = (uint64_t)c.kx + (uint64_t)g_h1 * (uint64_t)c.k2;
+        }
+        return c;
     };
     auto at = [&](const Ctx& c, int n) { return reinterpret_cast<cplx<T>*>(c.row + (uint64_t)(uint32_t)n * Sb); };
     auto gld = [&](const Ctx& c, int n) -> cplx<T> {
@@ -125,32 +151,27 @@ __global__ __launch_bounds__(kBlock) void k_h_strided(const FieldH* __restrict__
     // gpair falls back to one gld per point on those planes, for host-exported noise and for a
     // workspace source.
     auto gpair_dev = [&](const Ctx& c, int q, bool has_hi, cplx<T>& lo, cplx<T>& hi) {
-        const int N = pl.N, h3 = (N + 1) >> 1;
+        const int N = g_N, h3 = (N + 1) >> 1;
         {
             if constexpr (sizeof(T) == 4) {
                 // amp(k) W(k) = sqrt(-ln2 amp0^2 log2(u1) P(q)) (cos 2 pi u2, sin 2 pi u2) with
                 // P = (1 + s)^(2 expo) (Matern family) or 2^(-s / (4 ln 2)) (Gaussian), s = sum_d
                 // (alpha_d f_d)^2: amplitude, the 1/sqrt2 of the complex draw and the Box-Muller
-                // radius share one square root; |f_z| = min(kz, n3 - kz).
-                const float ax = float(f.alpha[0]) * float(c.kx);
-                const float ay = float(f.alpha[1]) * float(signed_freq(c.k2, f.n[1]));
-                const bool gauss = f.model == CM_GAUSSIAN;
-                const float qxy = fmaf(ax, ax, ay * ay) + (gauss ? 0.0f : 1.0f);
-                const float az = float(f.alpha[2]);
-                const float e2 = gauss ? -0.36067376022224085f : float(2.0 * f.expo);
-                const float cc = float(-0.69314718055994530942 * f.amp0 * f.amp0);
-                auto root = [&](int kz, uint32_t u) {
-                    const float t = az * float(min(kz, N - kz));
-                    const float s2 = fmaf(t, t, qxy);
-                    const float P = __builtin_amdgcn_exp2f(e2 * (gauss ? s2 : __builtin_amdgcn_logf(s2)));
-                    return __builtin_amdgcn_sqrtf(cc * __builtin_amdgcn_logf(u24<float>(u)) * P);
+                // radius share one square root.  |f_z| = min(kz, n3 - kz) is q for kz = q < h3 and
+                // n3 - h3 - q for kz = q + h3.
+                auto root = [&](float fz, uint32_t u) {
+                    const float t = g_az * fz;
+                    const float s2 = fmaf(t, t, c.qxy);
+                    const float P = __builtin_amdgcn_exp2f(g_e2 * (gauss ? s2 : __builtin_amdgcn_logf(s2)));
+                    return __builtin_amdgcn_sqrtf(g_cc * __builtin_amdgcn_logf(u24<float>(u)) * P);
                 };
                 uint32_t o[4];
-                philox_at(c.kx + (int64_t)f.h1 * (c.k2 + (int64_t)f.n[1] * q), f.seed, f.stream, o);
-                const float r0 = root(q, o[0]), v0 = u24<float>(o[1]);
+                philox_at(c.ctr + (uint64_t)g_cstride * (uint32_t)q, g_seed, g_stream, o);
+                const float fq = float(q);
+                const float r0 = root(fq, o[0]), v0 = u24<float>(o[1]);
                 lo = cplx<T>{r0 * __builtin_amdgcn_cosf(v0), r0 * __builtin_amdgcn_sinf(v0)};
                 if (has_hi) {
-                    const float r1 = root(q + h3, o[2]), v1 = u24<float>(o[3]);
+                    const float r1 = root(float(N - h3) - fq, o[2]), v1 = u24<float>(o[3]);
                     hi = cplx<T>{r1 * __builtin_amdgcn_cosf(v1), r1 * __builtin_amdgcn_sinf(v1)};
                 }
             } else {
@@ -166,8 +187,8 @@ __global__ __launch_bounds__(kBlock) void k_h_strided(const FieldH* __restrict__
         }
         // a kx = 0 pencil (self-conjugate: the draw above pairs the wrong points) takes the inputs
         // k_h_kx0 made; every lane draws, so the generator stays out of any branch
-        if (f.kx0 != nullptr && c.kx == 0) {
-            const cplx<T>* z0 = reinterpret_cast<const cplx<T>*>(f.kx0) + (int64_t)f.n[2] * c.k2;
+        if (g_kx0 != nullptr && c.kx == 0) {
+            const cplx<T>* z0 = g_kx0 + (int64_t)n3 * c.k2;
             lo = gload(z0 + q);
             if (has_hi) hi = gload(z0 + (q + h3));
         }
