@@ -128,6 +128,8 @@ struct Context {
     int device = 0;
     hipStream_t stream = nullptr;
     hipStream_t gen_stream = nullptr;  // generating passes of a pipelined batch (default priority)
+    // the blend's x launches of one colour beside each other (default priority)
+    hipStream_t side[2] = {nullptr, nullptr};
     bool timing = false;
     std::vector<TimedLaunch> pending;
     std::map<std::string, std::pair<long, double>> totals;
@@ -169,6 +171,7 @@ static int ensure_init() {
     (void)noalloc_setup(g_ctx.device);
     HIP_TRY(hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking));
     HIP_TRY(hipStreamCreateWithFlags(&g_ctx.gen_stream, hipStreamNonBlocking));
+    for (hipStream_t& st : g_ctx.side) HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
     g_ctx.ready = true;
     return 0;
 }
@@ -841,6 +844,7 @@ struct scarl_batch {
     std::vector<scarl::Tile> tz, ty, tx;     // tiles of the z, y and x passes
     struct TileClass {
         int plan, t0, t1;                    // tiles [t0, t1) of one plan (one kernel launch)
+        int color = 0;                       // x pass of a blend: the colour of its subdomains
     };
     // A group of fields runs its z, y and x passes in order; in a pipelined batch the z pass of
     // group g + 1 (compute-bound generation) overlaps the y and x passes of group g (HBM-bound).
@@ -849,6 +853,7 @@ struct scarl_batch {
     };
     std::vector<Group> groups;
     std::vector<hipEvent_t> gev;             // fork event, then one per group (pipelined batches)
+    hipEvent_t xev[3] = {nullptr, nullptr, nullptr};  // blend x pass: fork, joins of the side streams
     std::vector<int2> field_xtiles;          // per-field [begin, end) in tx
     scarl::FieldH* d_fields = nullptr;
     scarl::Tile *d_tz = nullptr, *d_ty = nullptr, *d_tx = nullptr;
@@ -871,6 +876,8 @@ struct scarl_batch {
         if (gexec) hipGraphExecDestroy(gexec);
         if (graph) hipGraphDestroy(graph);
         for (hipEvent_t e : gev) hipEventDestroy(e);
+        for (hipEvent_t e : xev)
+            if (e) hipEventDestroy(e);
         for (auto& f : fields)
             if (f.noise_buf) hipFree(f.noise_buf);
         void* ptrs[] = {d_fields, d_tz, d_ty, d_tx, d_field_xtiles, d_partials, d_field_mom, d_slot_count, d_slot_out,
@@ -1097,7 +1104,7 @@ static void make_tiles(scarl_batch* b) {
                     for (int64_t t = (rows + T - 1) / T - 1; t >= 0; --t) b->tx.push_back({i, (int)(t * T)});
                     b->field_xtiles[i].y = (int)b->tx.size();
                 }
-                g.cx.push_back({kv.first, t0, (int)b->tx.size()});
+                g.cx.push_back({kv.first, t0, (int)b->tx.size(), c});
             }
         }
         b->groups.push_back(std::move(g));
@@ -1126,6 +1133,8 @@ static int upload_tables(scarl_batch* b) {
             HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
             b->gev.push_back(e);
         }
+    if (b->global && !b->xev[0])
+        for (hipEvent_t& e : b->xev) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     for (const PlanDev& pl : b->ps.plans) SCARL_TRY(check_lds((size_t)pl.lds_elems * 2 * rsz));
     void** tabs[] = {(void**)&b->d_tz, (void**)&b->d_ty, (void**)&b->d_tx, (void**)&b->d_field_xtiles,
                      (void**)&b->d_partials};
@@ -1232,17 +1241,40 @@ static int enqueue_group(scarl_batch* b, const scarl_batch::Group& g, hipStream_
                                     b->ps.d_plans, b->ps.d_pool, (size_t)pl.lds_elems * csz, s);
         }));
     }
-    for (const auto& cl : g.cx) {
-        if (only_plan >= 0 && cl.plan != only_plan) continue;
-        const PlanDev& pl = b->ps.plans[cl.plan];
-        // x-pass plans of odd n1 are plain C2C (no pre-twiddle table); a blend keeps its x weights
-        // (n1 reals) in LDS after the transform's tile
-        const bool full = pl.pre_off < 0;
-        const size_t lds = (size_t)pl.lds_elems * csz + (b->global ? (size_t)(full ? pl.N : 2 * pl.N) * (csz / 2) : 0);
-        SCARL_TRY(timed("h_xpass", [&] {
-            return launch_h_xpass(f64, pl.spec, pl.ecap, b->global, full, b->d_fields, b->d_tx, cl.t0, cl.t1 - cl.t0,
-                                  b->ps.d_plans, b->ps.d_pool, b->d_partials, lds, s);
-        }));
+    // A blend's x launches go colour by colour (a later owner of an overlap point adds to what
+    // the earlier ones stored).  The launches of one colour (one per plan) touch disjoint
+    // subdomains, so they run side by side on the library stream and two side streams, which
+    // fills the GPU through each launch's tail; with event timing on, every launch runs alone.
+    const bool fan = b->global && !g_ctx.timing && only_plan < 0 && b->xev[0] != nullptr;
+    for (size_t i = 0; i < g.cx.size();) {
+        size_t e = i + 1;
+        while (e < g.cx.size() && g.cx[e].color == g.cx[i].color) ++e;
+        const int nside = fan ? (int)std::min<size_t>(2, e - i - 1) : 0;
+        if (nside > 0) {
+            HIP_TRY(hipEventRecord(b->xev[0], s));
+            for (int k = 0; k < nside; ++k) HIP_TRY(hipStreamWaitEvent(g_ctx.side[k], b->xev[0], 0));
+        }
+        for (size_t c = i; c < e; ++c) {
+            const auto& cl = g.cx[c];
+            if (only_plan >= 0 && cl.plan != only_plan) continue;
+            const PlanDev& pl = b->ps.plans[cl.plan];
+            // x-pass plans of odd n1 are plain C2C (no pre-twiddle table); a blend keeps its x
+            // weights (n1 reals) in LDS after the transform's tile
+            const bool full = pl.pre_off < 0;
+            const size_t lds =
+                (size_t)pl.lds_elems * csz + (b->global ? (size_t)(full ? pl.N : 2 * pl.N) * (csz / 2) : 0);
+            const int lane = nside > 0 ? (int)((c - i) % (size_t)(nside + 1)) : 0;
+            hipStream_t xs = lane == 0 ? s : g_ctx.side[lane - 1];
+            SCARL_TRY(timed("h_xpass", [&] {
+                return launch_h_xpass(f64, pl.spec, pl.ecap, b->global, full, b->d_fields, b->d_tx, cl.t0,
+                                      cl.t1 - cl.t0, b->ps.d_plans, b->ps.d_pool, b->d_partials, lds, xs);
+            }));
+        }
+        for (int k = 0; k < nside; ++k) {
+            HIP_TRY(hipEventRecord(b->xev[1 + k], g_ctx.side[k]));
+            HIP_TRY(hipStreamWaitEvent(s, b->xev[1 + k], 0));
+        }
+        i = e;
     }
     return 0;
 }
@@ -1892,6 +1924,10 @@ int scarl_hip_finalize(void) {
     }
     hipStreamDestroy(g_ctx.stream);
     hipStreamDestroy(g_ctx.gen_stream);
+    for (hipStream_t& st : g_ctx.side) {
+        hipStreamDestroy(st);
+        st = nullptr;
+    }
     g_ctx.stream = nullptr;
     g_ctx.gen_stream = nullptr;
     g_ctx.ready = false;
