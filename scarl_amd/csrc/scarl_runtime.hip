@@ -128,7 +128,8 @@ struct Context {
     int device = 0;
     hipStream_t stream = nullptr;
     hipStream_t gen_stream = nullptr;  // generating passes of a pipelined batch (default priority)
-    // the blend's x launches of one colour beside each other (default priority)
+    // launches of one pass beside each other in batches that do not pipeline (default priority;
+    // with the two above, the four hardware queues of the process)
     hipStream_t side[2] = {nullptr, nullptr};
     bool timing = false;
     std::vector<TimedLaunch> pending;
@@ -853,7 +854,7 @@ struct scarl_batch {
     };
     std::vector<Group> groups;
     std::vector<hipEvent_t> gev;             // fork event, then one per group (pipelined batches)
-    hipEvent_t xev[3] = {nullptr, nullptr, nullptr};  // blend x pass: fork, joins of the side streams
+    hipEvent_t xev[4] = {nullptr, nullptr, nullptr, nullptr};  // fan-out of a pass: fork, three joins
     std::vector<int2> field_xtiles;          // per-field [begin, end) in tx
     scarl::FieldH* d_fields = nullptr;
     scarl::Tile *d_tz = nullptr, *d_ty = nullptr, *d_tx = nullptr;
@@ -1133,7 +1134,7 @@ static int upload_tables(scarl_batch* b) {
             HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
             b->gev.push_back(e);
         }
-    if (b->global && !b->xev[0])
+    if (b->groups.size() == 1 && !b->xev[0])
         for (hipEvent_t& e : b->xev) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     for (const PlanDev& pl : b->ps.plans) SCARL_TRY(check_lds((size_t)pl.lds_elems * 2 * rsz));
     void** tabs[] = {(void**)&b->d_tz, (void**)&b->d_ty, (void**)&b->d_tx, (void**)&b->d_field_xtiles,
@@ -1221,59 +1222,64 @@ static int enqueue_group(scarl_batch* b, const scarl_batch::Group& g, hipStream_
                          hipEvent_t z_done, int only_plan = -1) {
     const bool f64 = b->fp64;
     const size_t csz = f64 ? 16 : 8;
-    for (const auto& cl : g.cz) {
-        if (only_plan >= 0 && cl.plan != only_plan) continue;
+    // The launches of one pass (one per plan; in a blend's x pass, those of one colour, since a
+    // later owner of an overlap point adds to what the earlier ones stored) touch disjoint fields
+    // or subdomains.  In a batch that does not pipeline they run side by side on the stream of the
+    // pass and up to three more (the generating stream is free there), which fills the GPU
+    // through each launch's tail.  With event timing on, every launch runs alone.
+    const bool fan = z_done == nullptr && !g_ctx.timing && only_plan < 0 && b->xev[0] != nullptr;
+    using TC = scarl_batch::TileClass;
+    auto fan_out = [&](const std::vector<TC>& cls, size_t i, size_t e, hipStream_t home, auto&& launch) -> int {
+        hipStream_t side[3] = {g_ctx.gen_stream, g_ctx.side[0], g_ctx.side[1]};
+        const int nside = fan ? (int)std::min<size_t>(3, e - i - 1) : 0;
+        if (nside > 0) {
+            HIP_TRY(hipEventRecord(b->xev[0], home));
+            for (int k = 0; k < nside; ++k) HIP_TRY(hipStreamWaitEvent(side[k], b->xev[0], 0));
+        }
+        for (size_t c = i; c < e; ++c) {
+            if (only_plan >= 0 && cls[c].plan != only_plan) continue;
+            const int lane = nside > 0 ? (int)((c - i) % (size_t)(nside + 1)) : 0;
+            SCARL_TRY(launch(cls[c], lane == 0 ? home : side[lane - 1]));
+        }
+        for (int k = 0; k < nside; ++k) {
+            HIP_TRY(hipEventRecord(b->xev[1 + k], side[k]));
+            HIP_TRY(hipStreamWaitEvent(home, b->xev[1 + k], 0));
+        }
+        return 0;
+    };
+    SCARL_TRY(fan_out(g.cz, 0, g.cz.size(), zs, [&](const TC& cl, hipStream_t st) {
         const PlanDev& pl = b->ps.plans[cl.plan];
-        SCARL_TRY(timed("h_zpass", [&] {
+        return timed("h_zpass", [&] {
             return launch_h_strided(f64, 2, true, pl.spec, pl.ecap, b->d_fields, b->d_tz, cl.t0, cl.t1 - cl.t0,
-                                    b->ps.d_plans, b->ps.d_pool, (size_t)pl.lds_elems * csz, zs);
-        }));
-    }
+                                    b->ps.d_plans, b->ps.d_pool, (size_t)pl.lds_elems * csz, st);
+        });
+    }));
     if (z_done) {
         HIP_TRY(hipEventRecord(z_done, zs));
         HIP_TRY(hipStreamWaitEvent(s, z_done, 0));
     }
-    for (const auto& cl : g.cy) {
-        if (only_plan >= 0 && cl.plan != only_plan) continue;
+    SCARL_TRY(fan_out(g.cy, 0, g.cy.size(), s, [&](const TC& cl, hipStream_t st) {
         const PlanDev& pl = b->ps.plans[cl.plan];
-        SCARL_TRY(timed("h_ypass", [&] {
+        return timed("h_ypass", [&] {
             return launch_h_strided(f64, 1, b->any_2d, pl.spec, pl.ecap, b->d_fields, b->d_ty, cl.t0, cl.t1 - cl.t0,
-                                    b->ps.d_plans, b->ps.d_pool, (size_t)pl.lds_elems * csz, s);
-        }));
-    }
-    // A blend's x launches go colour by colour (a later owner of an overlap point adds to what
-    // the earlier ones stored).  The launches of one colour (one per plan) touch disjoint
-    // subdomains, so they run side by side on the library stream and two side streams, which
-    // fills the GPU through each launch's tail; with event timing on, every launch runs alone.
-    const bool fan = b->global && !g_ctx.timing && only_plan < 0 && b->xev[0] != nullptr;
+                                    b->ps.d_plans, b->ps.d_pool, (size_t)pl.lds_elems * csz, st);
+        });
+    }));
     for (size_t i = 0; i < g.cx.size();) {
         size_t e = i + 1;
         while (e < g.cx.size() && g.cx[e].color == g.cx[i].color) ++e;
-        const int nside = fan ? (int)std::min<size_t>(2, e - i - 1) : 0;
-        if (nside > 0) {
-            HIP_TRY(hipEventRecord(b->xev[0], s));
-            for (int k = 0; k < nside; ++k) HIP_TRY(hipStreamWaitEvent(g_ctx.side[k], b->xev[0], 0));
-        }
-        for (size_t c = i; c < e; ++c) {
-            const auto& cl = g.cx[c];
-            if (only_plan >= 0 && cl.plan != only_plan) continue;
+        SCARL_TRY(fan_out(g.cx, i, e, s, [&](const TC& cl, hipStream_t st) {
             const PlanDev& pl = b->ps.plans[cl.plan];
             // x-pass plans of odd n1 are plain C2C (no pre-twiddle table); a blend keeps its x
             // weights (n1 reals) in LDS after the transform's tile
             const bool full = pl.pre_off < 0;
             const size_t lds =
                 (size_t)pl.lds_elems * csz + (b->global ? (size_t)(full ? pl.N : 2 * pl.N) * (csz / 2) : 0);
-            const int lane = nside > 0 ? (int)((c - i) % (size_t)(nside + 1)) : 0;
-            hipStream_t xs = lane == 0 ? s : g_ctx.side[lane - 1];
-            SCARL_TRY(timed("h_xpass", [&] {
+            return timed("h_xpass", [&] {
                 return launch_h_xpass(f64, pl.spec, pl.ecap, b->global, full, b->d_fields, b->d_tx, cl.t0,
-                                      cl.t1 - cl.t0, b->ps.d_plans, b->ps.d_pool, b->d_partials, lds, xs);
-            }));
-        }
-        for (int k = 0; k < nside; ++k) {
-            HIP_TRY(hipEventRecord(b->xev[1 + k], g_ctx.side[k]));
-            HIP_TRY(hipStreamWaitEvent(s, b->xev[1 + k], 0));
-        }
+                                      cl.t1 - cl.t0, b->ps.d_plans, b->ps.d_pool, b->d_partials, lds, st);
+            });
+        }));
         i = e;
     }
     return 0;
