@@ -480,61 +480,55 @@ __global__ __launch_bounds__(kBlock) void k_h_xpass(const FieldH* __restrict__ f
 // and the pass's layout are instantiated).
 template <typename T, int PREC, int E1, int E2>
 struct ModeHDispatch {
-    template <class F>
-    static void strided(int axis, bool gen, const FieldH* fields, const Tile* tiles, int tile0, int ntiles,
+    template <bool GEN, class F>
+    static void strided(int axis, const FieldH* fields, const Tile* tiles, int tile0, int ntiles,
                         const PlanDev* plans, const cplx<T>* p, size_t lds, hipStream_t s) {
-        if (gen)
-            hipLaunchKernelGGL((k_h_strided<T, true, F>), dim3(ntiles), dim3(kBlock), lds, s, fields, tiles, tile0,
-                               plans, p, axis);
-        else
-            hipLaunchKernelGGL((k_h_strided<T, false, F>), dim3(ntiles), dim3(kBlock), lds, s, fields, tiles, tile0,
-                               plans, p, axis);
+        hipLaunchKernelGGL((k_h_strided<T, GEN, F>), dim3(ntiles), dim3(kBlock), lds, s, fields, tiles, tile0, plans,
+                           p, axis);
     }
-    template <class F, bool FULL = false>
-    static void xpass(bool blend, const FieldH* fields, const Tile* tiles, int tile0, int ntiles,
-                      const PlanDev* plans, const cplx<T>* p, MomPartial* partials, size_t lds, hipStream_t s) {
-        if (blend)
-            hipLaunchKernelGGL((k_h_xpass<T, F, true, FULL>), dim3(ntiles), dim3(kBlock), lds, s, fields, tiles, tile0,
-                               plans, p, partials);
-        else
-            hipLaunchKernelGGL((k_h_xpass<T, F, false, FULL>), dim3(ntiles), dim3(kBlock), lds, s, fields, tiles,
-                               tile0, plans, p, partials);
+    template <bool BLEND, class F, bool FULL = false>
+    static void xpass(const FieldH* fields, const Tile* tiles, int tile0, int ntiles, const PlanDev* plans,
+                      const cplx<T>* p, MomPartial* partials, size_t lds, hipStream_t s) {
+        hipLaunchKernelGGL((k_h_xpass<T, F, BLEND, FULL>), dim3(ntiles), dim3(kBlock), lds, s, fields, tiles, tile0,
+                           plans, p, partials);
     }
-    static hipError_t run_strided(int axis, bool gen, int spec, int ecap, const FieldH* fields, const Tile* tiles,
+    template <bool GEN>
+    static hipError_t run_strided(int axis, int spec, int ecap, const FieldH* fields, const Tile* tiles,
                                   int tile0, int ntiles, const PlanDev* plans, const void* pool, size_t lds,
                                   hipStream_t s) {
         if (ntiles <= 0) return hipSuccess;
         const cplx<T>* p = static_cast<const cplx<T>*>(pool);
         if (spec == kSpecMfmaGeneric) {
-            if constexpr (PREC == 4) strided<GenericMfmaFft<E1>>(axis, gen, fields, tiles, tile0, ntiles, plans, p, lds, s);
+            if constexpr (PREC == 4) strided<GEN, GenericMfmaFft<E1>>(axis, fields, tiles, tile0, ntiles, plans, p, lds, s);
             return hipGetLastError();
         }
         if (spec == kSpecNonSmooth) {
-            if (ecap == E2) strided<NonSmoothFft<E2>>(axis, gen, fields, tiles, tile0, ntiles, plans, p, lds, s);
-            else strided<NonSmoothFft<E1>>(axis, gen, fields, tiles, tile0, ntiles, plans, p, lds, s);
+            if (ecap == E2) strided<GEN, NonSmoothFft<E2>>(axis, fields, tiles, tile0, ntiles, plans, p, lds, s);
+            else strided<GEN, NonSmoothFft<E1>>(axis, fields, tiles, tile0, ntiles, plans, p, lds, s);
             return hipGetLastError();
         }
         if (spec == kSpecGenericComp) {
-            if (ecap == E2) strided<GenericCompFft<E2>>(axis, gen, fields, tiles, tile0, ntiles, plans, p, lds, s);
-            else strided<GenericCompFft<E1>>(axis, gen, fields, tiles, tile0, ntiles, plans, p, lds, s);
+            if (ecap == E2) strided<GEN, GenericCompFft<E2>>(axis, fields, tiles, tile0, ntiles, plans, p, lds, s);
+            else strided<GEN, GenericCompFft<E1>>(axis, fields, tiles, tile0, ntiles, plans, p, lds, s);
             return hipGetLastError();
         }
         switch (spec) {
 #define SCARL_CASE(ID, P, L, N, R0, R1, TPP)                                                             \
     case ID:                                                                                             \
         if constexpr (P == PREC && L == LAYOUT_PF)                                                       \
-            strided<FixedFft<N, R0, R1, TPP>>(axis, gen, fields, tiles, tile0, ntiles, plans, p, lds, s); \
+            strided<GEN, FixedFft<N, R0, R1, TPP>>(axis, fields, tiles, tile0, ntiles, plans, p, lds, s); \
         break;
             SCARL_FIXED_SPECS(SCARL_CASE)
 #undef SCARL_CASE
         default:
-            if (ecap == E2) strided<GenericFft<E2>>(axis, gen, fields, tiles, tile0, ntiles, plans, p, lds, s);
-            else strided<GenericFft<E1>>(axis, gen, fields, tiles, tile0, ntiles, plans, p, lds, s);
+            if (ecap == E2) strided<GEN, GenericFft<E2>>(axis, fields, tiles, tile0, ntiles, plans, p, lds, s);
+            else strided<GEN, GenericFft<E1>>(axis, fields, tiles, tile0, ntiles, plans, p, lds, s);
         }
         return hipGetLastError();
     }
     // full: odd n1, whose lengths only the generic kernel plans
-    static hipError_t run_xpass(int spec, int ecap, bool blend, bool full, const FieldH* fields, const Tile* tiles,
+    template <bool BLEND>
+    static hipError_t run_xpass(int spec, int ecap, bool full, const FieldH* fields, const Tile* tiles,
                                 int tile0, int ntiles, const PlanDev* plans, const void* pool, MomPartial* partials,
                                 size_t lds, hipStream_t s) {
         if (ntiles <= 0) return hipSuccess;
@@ -542,47 +536,47 @@ struct ModeHDispatch {
         const cplx<T>* p = static_cast<const cplx<T>*>(pool);
         if (full) {
             if (spec == kSpecNonSmooth) {
-                if (ecap == E2) xpass<NonSmoothFft<E2>, true>(blend, fields, tiles, tile0, ntiles, plans, p, partials, lds, s);
-                else xpass<NonSmoothFft<E1>, true>(blend, fields, tiles, tile0, ntiles, plans, p, partials, lds, s);
+                if (ecap == E2) xpass<BLEND, NonSmoothFft<E2>, true>(fields, tiles, tile0, ntiles, plans, p, partials, lds, s);
+                else xpass<BLEND, NonSmoothFft<E1>, true>(fields, tiles, tile0, ntiles, plans, p, partials, lds, s);
             } else if (spec == kSpecGenericComp) {
-                if (ecap == E2) xpass<GenericCompFft<E2>, true>(blend, fields, tiles, tile0, ntiles, plans, p, partials, lds, s);
-                else xpass<GenericCompFft<E1>, true>(blend, fields, tiles, tile0, ntiles, plans, p, partials, lds, s);
+                if (ecap == E2) xpass<BLEND, GenericCompFft<E2>, true>(fields, tiles, tile0, ntiles, plans, p, partials, lds, s);
+                else xpass<BLEND, GenericCompFft<E1>, true>(fields, tiles, tile0, ntiles, plans, p, partials, lds, s);
             } else if (ecap == E2) {
-                xpass<GenericFft<E2>, true>(blend, fields, tiles, tile0, ntiles, plans, p, partials, lds, s);
+                xpass<BLEND, GenericFft<E2>, true>(fields, tiles, tile0, ntiles, plans, p, partials, lds, s);
             } else {
-                xpass<GenericFft<E1>, true>(blend, fields, tiles, tile0, ntiles, plans, p, partials, lds, s);
+                xpass<BLEND, GenericFft<E1>, true>(fields, tiles, tile0, ntiles, plans, p, partials, lds, s);
             }
             return hipGetLastError();
         }
         if (spec == kSpecNonSmooth) {
-            if (ecap == E2) xpass<NonSmoothFft<E2>>(blend, fields, tiles, tile0, ntiles, plans, p, partials, lds, s);
-            else xpass<NonSmoothFft<E1>>(blend, fields, tiles, tile0, ntiles, plans, p, partials, lds, s);
+            if (ecap == E2) xpass<BLEND, NonSmoothFft<E2>>(fields, tiles, tile0, ntiles, plans, p, partials, lds, s);
+            else xpass<BLEND, NonSmoothFft<E1>>(fields, tiles, tile0, ntiles, plans, p, partials, lds, s);
             return hipGetLastError();
         }
         if (spec == kSpecGenericComp) {
-            if (ecap == E2) xpass<GenericCompFft<E2>>(blend, fields, tiles, tile0, ntiles, plans, p, partials, lds, s);
-            else xpass<GenericCompFft<E1>>(blend, fields, tiles, tile0, ntiles, plans, p, partials, lds, s);
+            if (ecap == E2) xpass<BLEND, GenericCompFft<E2>>(fields, tiles, tile0, ntiles, plans, p, partials, lds, s);
+            else xpass<BLEND, GenericCompFft<E1>>(fields, tiles, tile0, ntiles, plans, p, partials, lds, s);
             return hipGetLastError();
         }
         if (spec == kSpecMfma256) {
-            if constexpr (PREC == 4) xpass<MfmaFft256>(blend, fields, tiles, tile0, ntiles, plans, p, partials, lds, s);
+            if constexpr (PREC == 4) xpass<BLEND, MfmaFft256>(fields, tiles, tile0, ntiles, plans, p, partials, lds, s);
             return hipGetLastError();
         }
         if (spec == kSpecMfmaGeneric) {
-            if constexpr (PREC == 4) xpass<GenericMfmaFft<E1>>(blend, fields, tiles, tile0, ntiles, plans, p, partials, lds, s);
+            if constexpr (PREC == 4) xpass<BLEND, GenericMfmaFft<E1>>(fields, tiles, tile0, ntiles, plans, p, partials, lds, s);
             return hipGetLastError();
         }
         switch (spec) {
 #define SCARL_CASE(ID, P, L, N, R0, R1, TPP)                                                                  \
     case ID:                                                                                                  \
         if constexpr (P == PREC && L == LAYOUT_BF)                                                            \
-            xpass<FixedFft<N, R0, R1, TPP>>(blend, fields, tiles, tile0, ntiles, plans, p, partials, lds, s); \
+            xpass<BLEND, FixedFft<N, R0, R1, TPP>>(fields, tiles, tile0, ntiles, plans, p, partials, lds, s); \
         break;
             SCARL_FIXED_SPECS(SCARL_CASE)
 #undef SCARL_CASE
         default:
-            if (ecap == E2) xpass<GenericFft<E2>>(blend, fields, tiles, tile0, ntiles, plans, p, partials, lds, s);
-            else xpass<GenericFft<E1>>(blend, fields, tiles, tile0, ntiles, plans, p, partials, lds, s);
+            if (ecap == E2) xpass<BLEND, GenericFft<E2>>(fields, tiles, tile0, ntiles, plans, p, partials, lds, s);
+            else xpass<BLEND, GenericFft<E1>>(fields, tiles, tile0, ntiles, plans, p, partials, lds, s);
         }
         return hipGetLastError();
     }
@@ -591,13 +585,34 @@ struct ModeHDispatch {
 }  // namespace scarl
 
 // Instantiate the launch wrappers for one precision T (SFX suffix, PREC bytes, register budgets).
-#define SCARL_MODEH_WRAPPERS(T, SFX, PREC, E1, E2)                                                          \
+// Launch wrappers of one precision, in four translation units that compile in parallel: the
+// generating strided passes (SCARL_MODEH_STRIDED with GEN true), the other strided passes (GEN
+// false), the blend's x pass (SCARL_MODEH_XPASS_PART with BLEND true), and the plain x pass with
+// the kx = 0 plane and the dispatchers (SCARL_MODEH_XPASS and the part with BLEND false).
+#define SCARL_MODEH_STRIDED(T, SFX, PREC, E1, E2, GEN, TAG)                                                 \
     namespace scarl {                                                                                      \
+    hipError_t launch_h_strided_##TAG##_##SFX(int axis, int spec, int ecap, const FieldH* fields,          \
+                                              const Tile* tiles, int tile0, int ntiles, const PlanDev* plans, \
+                                              const void* pool, size_t lds, hipStream_t s) {               \
+        return ModeHDispatch<T, PREC, E1, E2>::template run_strided<GEN>(axis, spec, ecap, fields, tiles, tile0, \
+                                                                         ntiles, plans, pool, lds, s);     \
+    }                                                                                                      \
+    }
+#define SCARL_MODEH_XPASS(T, SFX, PREC, E1, E2)                                                             \
+    namespace scarl {                                                                                      \
+    hipError_t launch_h_strided_gen_##SFX(int axis, int spec, int ecap, const FieldH* fields, const Tile* tiles, \
+                                          int tile0, int ntiles, const PlanDev* plans, const void* pool,   \
+                                          size_t lds, hipStream_t s);                                      \
+    hipError_t launch_h_strided_plain_##SFX(int axis, int spec, int ecap, const FieldH* fields,            \
+                                            const Tile* tiles, int tile0, int ntiles, const PlanDev* plans, \
+                                            const void* pool, size_t lds, hipStream_t s);                  \
     hipError_t launch_h_strided_##SFX(int axis, bool gen, int spec, int ecap, const FieldH* fields,        \
                                       const Tile* tiles, int tile0, int ntiles, const PlanDev* plans,      \
                                       const void* pool, size_t lds, hipStream_t s) {                       \
-        return ModeHDispatch<T, PREC, E1, E2>::run_strided(axis, gen, spec, ecap, fields, tiles, tile0, ntiles, \
-                                                           plans, pool, lds, s);                           \
+        if (ntiles <= 0) return hipSuccess;                                                                \
+        return gen ? launch_h_strided_gen_##SFX(axis, spec, ecap, fields, tiles, tile0, ntiles, plans, pool, lds, s) \
+                   : launch_h_strided_plain_##SFX(axis, spec, ecap, fields, tiles, tile0, ntiles, plans, pool, lds, \
+                                                  s);                                                      \
     }                                                                                                      \
     hipError_t launch_h_kx0_##SFX(const FieldH* fields, int nfields, int64_t max_pts, hipStream_t s) {     \
         if (nfields <= 0 || max_pts <= 0) return hipSuccess;                                               \
@@ -605,10 +620,27 @@ struct ModeHDispatch {
                            dim3(kBlock), 0, s, fields);                                                     \
         return hipGetLastError();                                                                          \
     }                                                                                                      \
+    hipError_t launch_h_xpass_blend_##SFX(int spec, int ecap, bool full, const FieldH* fields, const Tile* tiles, \
+                                          int tile0, int ntiles, const PlanDev* plans, const void* pool,   \
+                                          MomPartial* partials, size_t lds, hipStream_t s);                \
+    hipError_t launch_h_xpass_plain_##SFX(int spec, int ecap, bool full, const FieldH* fields, const Tile* tiles, \
+                                          int tile0, int ntiles, const PlanDev* plans, const void* pool,   \
+                                          MomPartial* partials, size_t lds, hipStream_t s);                \
     hipError_t launch_h_xpass_##SFX(int spec, int ecap, bool blend, bool full, const FieldH* fields,       \
                                     const Tile* tiles, int tile0, int ntiles, const PlanDev* plans,        \
                                     const void* pool, MomPartial* partials, size_t lds, hipStream_t s) {   \
-        return ModeHDispatch<T, PREC, E1, E2>::run_xpass(spec, ecap, blend, full, fields, tiles, tile0, ntiles, \
-                                                         plans, pool, partials, lds, s);                   \
+        return blend ? launch_h_xpass_blend_##SFX(spec, ecap, full, fields, tiles, tile0, ntiles, plans, pool, \
+                                                  partials, lds, s)                                        \
+                     : launch_h_xpass_plain_##SFX(spec, ecap, full, fields, tiles, tile0, ntiles, plans, pool, \
+                                                  partials, lds, s);                                       \
+    }                                                                                                      \
+    }
+#define SCARL_MODEH_XPASS_PART(T, SFX, PREC, E1, E2, BLEND, TAG)                                            \
+    namespace scarl {                                                                                      \
+    hipError_t launch_h_xpass_##TAG##_##SFX(int spec, int ecap, bool full, const FieldH* fields, const Tile* tiles, \
+                                            int tile0, int ntiles, const PlanDev* plans, const void* pool, \
+                                            MomPartial* partials, size_t lds, hipStream_t s) {             \
+        return ModeHDispatch<T, PREC, E1, E2>::template run_xpass<BLEND>(spec, ecap, full, fields, tiles, tile0, \
+                                                                         ntiles, plans, pool, partials, lds, s); \
     }                                                                                                      \
     }

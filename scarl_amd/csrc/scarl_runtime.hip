@@ -1034,9 +1034,13 @@ static void strided_tiles(int field, const FieldH& f, int nother, int T, int kin
 // Minimum z-pass tiles of a pipeline group: enough workgroups to fill the GPU several times over.
 constexpr int64_t kGroupTiles = 4096;
 
-// Tile tables.  Fields are cut into consecutive groups of at least kGroupTiles z-pass tiles (a
-// blended batch is one group: its x pass runs colour by colour over every subdomain).  Within a
-// group every pass has one launch per plan, each with its own kernel and LDS size; the costlier
+// Tile tables.  Fields are cut into consecutive groups of at least kGroupTiles z-pass tiles, so
+// that a group's workspace (about 170 MB for 4096 tiles) is still in the 256 MB Infinity Cache
+// when its y and x passes read it.  A blend's subdomains are ordered by colour and a group holds
+// one colour only: the x passes then run in colour order, as the owner order of the overlap
+// points requires, and the groups pipeline like an ensemble's.  Within a group every pass has one
+// launch per plan (for a blend, per plan of the group's colour), each with its own kernel and LDS
+// size; the costlier
 // self-conjugate tiles go first so that the round-robin dispatch of workgroups spreads them evenly
 // over the XCDs (in a row-major order every row's kx = 0 tile would fall on the same XCD whenever
 // the tiles per row are a multiple of 8).
@@ -1052,16 +1056,18 @@ static void make_tiles(scarl_batch* b) {
     // of its launches covers many tiles (the order of the outputs does not change)
     std::vector<int> order(b->fields.size());
     for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
-    if (!b->global)
-        std::stable_sort(order.begin(), order.end(), [&](int u, int v) {
-            const FieldH &fu = b->fields[u].f, &fv = b->fields[v].f;
-            if (fu.plan_z != fv.plan_z) return fu.plan_z < fv.plan_z;
-            if (fu.plan_y != fv.plan_y) return fu.plan_y < fv.plan_y;
-            return fu.plan_x < fv.plan_x;
-        });
+    std::stable_sort(order.begin(), order.end(), [&](int u, int v) {
+        const FieldH &fu = b->fields[u].f, &fv = b->fields[v].f;
+        if (b->global && (fu.color & 7) != (fv.color & 7)) return (fu.color & 7) < (fv.color & 7);
+        if (fu.plan_z != fv.plan_z) return fu.plan_z < fv.plan_z;
+        if (fu.plan_y != fv.plan_y) return fu.plan_y < fv.plan_y;
+        return fu.plan_x < fv.plan_x;
+    });
     for (int i : order) {
         const FieldH& f = b->fields[i].f;
-        if (!b->global && acc >= kGroupTiles) {
+        const bool new_colour = b->global && !members.back().empty() &&
+                                (b->fields[members.back().back()].f.color & 7) != (f.color & 7);
+        if (acc >= kGroupTiles || new_colour) {
             members.emplace_back();
             acc = 0;
         }
@@ -1134,7 +1140,7 @@ static int upload_tables(scarl_batch* b) {
             HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
             b->gev.push_back(e);
         }
-    if (b->groups.size() == 1 && !b->xev[0])
+    if (!b->xev[0])
         for (hipEvent_t& e : b->xev) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     for (const PlanDev& pl : b->ps.plans) SCARL_TRY(check_lds((size_t)pl.lds_elems * 2 * rsz));
     void** tabs[] = {(void**)&b->d_tz, (void**)&b->d_ty, (void**)&b->d_tx, (void**)&b->d_field_xtiles,
@@ -1227,11 +1233,16 @@ static int enqueue_group(scarl_batch* b, const scarl_batch::Group& g, hipStream_
     // or subdomains.  In a batch that does not pipeline they run side by side on the stream of the
     // pass and up to three more (the generating stream is free there), which fills the GPU
     // through each launch's tail.  With event timing on, every launch runs alone.
-    const bool fan = z_done == nullptr && !g_ctx.timing && only_plan < 0 && b->xev[0] != nullptr;
+    // In a pipelined batch the generating stream carries the next group's z pass, so only the x
+    // launches fan out, over the two side streams.
+    const bool pipelined = z_done != nullptr;
+    const bool fan = !g_ctx.timing && only_plan < 0 && b->xev[0] != nullptr;
     using TC = scarl_batch::TileClass;
     auto fan_out = [&](const std::vector<TC>& cls, size_t i, size_t e, hipStream_t home, auto&& launch) -> int {
-        hipStream_t side[3] = {g_ctx.gen_stream, g_ctx.side[0], g_ctx.side[1]};
-        const int nside = fan ? (int)std::min<size_t>(3, e - i - 1) : 0;
+        hipStream_t side3[3] = {g_ctx.gen_stream, g_ctx.side[0], g_ctx.side[1]};
+        hipStream_t* side = pipelined ? side3 + 1 : side3;
+        const bool use = fan && (!pipelined || &cls == &g.cx);
+        const int nside = use ? (int)std::min<size_t>(pipelined ? 2 : 3, e - i - 1) : 0;
         if (nside > 0) {
             HIP_TRY(hipEventRecord(b->xev[0], home));
             for (int k = 0; k < nside; ++k) HIP_TRY(hipStreamWaitEvent(side[k], b->xev[0], 0));
