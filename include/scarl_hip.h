@@ -51,14 +51,26 @@ int scarl_hip_get_stream(void** stream);
  * 2 does so in the x passes only; 0 (default) keeps the FP32 Stockham butterflies.
  * SCARL_TUNE_AUTOTUNE: 1 makes batch creation time every plan's candidates (compile-time kernel,
  * runtime radix splits with 16 or 32 elements per thread in either order, pencils per tile, LDS
- * skew, MFMA first stage) on the batch's own passes and keep the fastest; picks are cached per
- * process, and in the file named by the SCARL_TUNE_CACHE environment variable when it is set
- * (the reference's counterpart is FFTW's planner, SRC_F/fftw_ScaRL.f90:378).
+ * skew, MFMA first stage) on the batch's own passes, confirm the best in replays of the whole step
+ * and keep the fastest (and, with SCARL_TUNE_SCHEDULE 0, the faster schedule); picks are cached
+ * per process and batch geometry, and in the file named by the SCARL_TUNE_CACHE environment
+ * variable when it is set (the reference's counterpart is FFTW's planner,
+ * SRC_F/fftw_ScaRL.f90:378).
  * SCARL_TUNE_LAG_FFT: 1 (default) computes the periodic lag sums of the corrL estimator by FFT
  * (|FFT|^2 and back, FP64, as SRC_F/Stat_ScaRL/statistics_RF.f90:353-448 does) wherever the length
  * has a Stockham plan and there are at least 8 lags; 0 uses the direct products from LDS.  The
- * bounded lags of the per-axis covariance always use the direct products. */
-enum { SCARL_TUNE_FP32_ELEMS = 1, SCARL_TUNE_MFMA_DFT16 = 2, SCARL_TUNE_AUTOTUNE = 3, SCARL_TUNE_LAG_FFT = 4 };
+ * bounded lags of the per-axis covariance always use the direct products.
+ * SCARL_TUNE_SCHEDULE: how a batch of several groups runs: 0 (default) pipelined when a group's
+ * workspace exceeds the 256 MB Infinity Cache, else one group after another (the autotuner times
+ * both and keeps the faster); 1 always pipelined (the next group's generating pass on a second
+ * stream beside this group's y and x passes); 2 always one group after another. */
+enum {
+    SCARL_TUNE_FP32_ELEMS = 1,
+    SCARL_TUNE_MFMA_DFT16 = 2,
+    SCARL_TUNE_AUTOTUNE = 3,
+    SCARL_TUNE_LAG_FFT = 4,
+    SCARL_TUNE_SCHEDULE = 5
+};
 int scarl_hip_set_tuning(int32_t key, int32_t value);
 
 /* ---- fftw_ScaRL drop-in ------------------------------------------------------------------ */

@@ -46,7 +46,7 @@ class FieldDesc(C.Structure):
 
 
 # scarl_hip_set_tuning keys
-TUNE_FP32_ELEMS, TUNE_MFMA_DFT16, TUNE_AUTOTUNE, TUNE_LAG_FFT = 1, 2, 3, 4
+TUNE_FP32_ELEMS, TUNE_MFMA_DFT16, TUNE_AUTOTUNE, TUNE_LAG_FFT, TUNE_SCHEDULE = 1, 2, 3, 4, 5
 
 # symbol -> (restype, argtypes); the list is the complete ABI of include/scarl_hip.h
 _vp, _i32, _i64, _d = C.c_void_p, C.c_int32, C.c_int64, C.c_double

@@ -249,6 +249,7 @@ def seeded_tiling(gdim, nb, overlap, choices, seed):
 def set_tuning(key, value):
     """Plan tuning for batches created afterwards: A.TUNE_FP32_ELEMS (16 or 32 complex elements
     per thread), A.TUNE_MFMA_DFT16 (1: FP32 radix-16 first stages as split-precision bf16 MFMA
-    products; 0: FP32 Stockham butterflies) or A.TUNE_AUTOTUNE (1: time every plan's candidates at
-    batch creation and keep the fastest)."""
+    products; 0: FP32 Stockham butterflies), A.TUNE_AUTOTUNE (1: time every plan's candidates at
+    batch creation and keep the fastest) or A.TUNE_SCHEDULE (batches of several groups: 0 by
+    Infinity Cache fit, 1 pipelined, 2 one group after another)."""
     A.call("scarl_hip_set_tuning", int(key), int(value))

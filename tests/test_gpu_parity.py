@@ -316,11 +316,14 @@ def test_ensemble_graph_replay_matches_eager(hip):
             b.sync()
 
 
-def test_pipelined_groups_match_single_fields(hip):
-    """Three 256^3 fields make two pipeline groups (z pass of one group overlapping the y and x
-    passes of the other on a second stream); every field and its moments must equal, bit for bit,
-    the same field generated alone, through the hipGraph and eagerly."""
-    from scarl_amd import generator
+@pytest.mark.parametrize("schedule", [1, 2, 0])
+def test_pipelined_groups_match_single_fields(hip, schedule):
+    """Three 256^3 fields make two groups, run pipelined (z pass of one group overlapping the y and
+    x passes of the other on a second stream), one after another with each pass's launches fanned
+    out, or as the cache-fit rule picks (sequential: the groups' mean workspace, 100 MB, fits the
+    Infinity Cache); every field and its moments must equal, bit for bit, the same field generated alone,
+    through the hipGraph and eagerly."""
+    from scarl_amd import _capi as A, generator
     specs = [generator.FieldSpec(n=(256, 256, 256), model=2, corrL=(16.0, 16.0, 16.0), nu=0.3, seed=5, stream=i,
                                  margi=2, avg=1.0, std_dev=0.3) for i in range(3)]
     alone, alone_mom = [], []
@@ -329,7 +332,14 @@ def test_pipelined_groups_match_single_fields(hip):
             s.run()
             alone.append(s.output(0))
             alone_mom.append(s.moments()[0])
-    with generator.Batch.ensemble(specs, 32) as b:
+    generator.set_tuning(A.TUNE_SCHEDULE, schedule)
+    try:
+        b = generator.Batch.ensemble(specs, 32)
+    finally:
+        generator.set_tuning(A.TUNE_SCHEDULE, 0)
+    with b:
+        rep = b.plan_report()
+        assert {r.get("schedule") for r in rep} == {"pipelined" if schedule == 1 else "sequential"}, rep
         for use_graph in (True, False, True):
             b.run(use_graph=use_graph)
             b.sync()
