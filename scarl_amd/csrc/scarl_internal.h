@@ -100,7 +100,8 @@ struct Tile {
 
 // Compile-time planned pass kernels: X(id, prec, layout, N, R0, R1, TPP); id 0 = generic runtime
 // plan.  prec: 4 = FP32, 8 = FP64; layout: LAYOUT_PF (strided y/z passes) or LAYOUT_BF (x pass);
-// a tile holds 256/TPP pencils and every tile stays within 64 KiB of LDS.
+// a tile holds 256/TPP pencils and every tile stays within 64 KiB of LDS.  The first kernel listed
+// for a length and layout is the default; the autotuner tries the others.
 #define SCARL_FIXED_SPECS(X)                \
     X(1, 4, LAYOUT_PF, 512, 16, 32, 16)     \
     X(2, 4, LAYOUT_PF, 256, 16, 16, 16)     \
@@ -112,6 +113,7 @@ struct Tile {
     X(8, 4, LAYOUT_BF, 256, 16, 16, 16)     \
     X(9, 4, LAYOUT_BF, 128, 16, 8, 8)       \
     X(10, 4, LAYOUT_BF, 64, 8, 8, 8)        \
+    X(24, 4, LAYOUT_BF, 96, 16, 6, 6)       \
     X(11, 4, LAYOUT_BF, 96, 32, 3, 4)       \
     X(12, 4, LAYOUT_BF, 80, 16, 5, 5)       \
     X(13, 4, LAYOUT_BF, 88, 8, 11, 8)       \
