@@ -60,10 +60,10 @@ int scarl_hip_get_stream(void** stream);
  * (|FFT|^2 and back, FP64, as SRC_F/Stat_ScaRL/statistics_RF.f90:353-448 does) wherever the length
  * has a Stockham plan and there are at least 8 lags; 0 uses the direct products from LDS.  The
  * bounded lags of the per-axis covariance always use the direct products.
- * SCARL_TUNE_SCHEDULE: how a batch of several groups runs: 0 (default) pipelined when a group's
- * workspace exceeds the 256 MB Infinity Cache, else one group after another (the autotuner times
- * both and keeps the faster); 1 always pipelined (the next group's generating pass on a second
- * stream beside this group's y and x passes); 2 always one group after another. */
+ * SCARL_TUNE_SCHEDULE: how a batch of several groups runs: 0 (default) pipelined for a blend or
+ * when the mean group workspace exceeds the 256 MB Infinity Cache, else one group after another
+ * (the autotuner times both and keeps the faster); 1 always pipelined (the next group's generating
+ * pass on a second stream beside this group's y and x passes); 2 always one group after another. */
 enum {
     SCARL_TUNE_FP32_ELEMS = 1,
     SCARL_TUNE_MFMA_DFT16 = 2,

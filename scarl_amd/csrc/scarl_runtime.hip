@@ -1170,8 +1170,10 @@ constexpr int64_t kMallBytes = int64_t(256) << 20;
 // (writing its own workspace beside them) evicts it: a 256-field ensemble of 64-192^3 (groups of
 // 62-282 MB, 150 MB on average) runs 4.57 ms per step pipelined and 4.15 ms one group after
 // another.  Groups larger than the cache (a 512^3 field each, 540 MB) have nothing to lose and
-// pipeline (2.23 against 2.26 ms).  The rule compares the mean group workspace with the cache; the
-// autotuner times both schedules and keeps the faster.
+// pipeline (2.23 against 2.26 ms).  A blend pipelines whatever its group size: its colour order
+// already serialises the x passes, and overlapping the next colour's generation with them pays
+// more than the cache (8 subdomains of 241^3: 3.81 against 4.03 ms; 1024 x 1024 x 256 from 128
+// subdomains: 2.51 against 2.53 ms).  The autotuner times both schedules and keeps the faster.
 static void choose_schedule(scarl_batch* b) {
     const size_t rsz = b->fp64 ? 8 : 4;
     int64_t total = 0;
@@ -1180,7 +1182,7 @@ static void choose_schedule(scarl_batch* b) {
         total += (f.wz * f.n[2] + (int64_t)f.nyq * f.n[1] * f.n[2]) * 2 * (int64_t)rsz;
     }
     const int64_t mean = total / std::max<int64_t>(1, (int64_t)b->groups.size());
-    b->pipeline = g_ctx.schedule == 0 ? mean > kMallBytes : g_ctx.schedule == 1;
+    b->pipeline = g_ctx.schedule == 0 ? (b->global || mean > kMallBytes) : g_ctx.schedule == 1;
 }
 
 static int finalize_batch(scarl_batch* b, const std::vector<double>& slot_count) {
