@@ -245,10 +245,10 @@ __global__ __launch_bounds__(kBlock) void k_h_strided(const FieldH* __restrict__
 // The kx = 0 plane's inputs a(k) W(k) of every 3D field with device noise (self-conjugate: the
 // draw of (0, ky, kz) is that of (0, -ky, -kz) conjugated), dense (n3, n2) with kz fastest (a z
 // pencil's inputs are contiguous), for the z pass's direct generation.  Block (x, y): points
-// [256 x, 256 x + 256) of field y.
+// [256 x, 256 x + 256) of field ids[y].
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_h_kx0(const FieldH* __restrict__ fields) {
-    const FieldH& f = fields[blockIdx.y];
+__global__ __launch_bounds__(kBlock) void k_h_kx0(const FieldH* __restrict__ fields, const int* __restrict__ ids) {
+    const FieldH& f = fields[ids[blockIdx.y]];
     if (f.kx0 == nullptr || f.src_mode != SRC_DEVICE_NOISE) return;
     const int n2 = f.n[1], n3 = f.n[2];
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -614,10 +614,11 @@ struct ModeHDispatch {
                    : launch_h_strided_plain_##SFX(axis, spec, ecap, fields, tiles, tile0, ntiles, plans, pool, lds, \
                                                   s);                                                      \
     }                                                                                                      \
-    hipError_t launch_h_kx0_##SFX(const FieldH* fields, int nfields, int64_t max_pts, hipStream_t s) {     \
+    hipError_t launch_h_kx0_##SFX(const FieldH* fields, const int* ids, int nfields, int64_t max_pts,      \
+                                  hipStream_t s) {                                                         \
         if (nfields <= 0 || max_pts <= 0) return hipSuccess;                                               \
         hipLaunchKernelGGL((k_h_kx0<T>), dim3((unsigned)((max_pts + kBlock - 1) / kBlock), (unsigned)nfields), \
-                           dim3(kBlock), 0, s, fields);                                                     \
+                           dim3(kBlock), 0, s, fields, ids);                                                \
         return hipGetLastError();                                                                          \
     }                                                                                                      \
     hipError_t launch_h_xpass_blend_##SFX(int spec, int ecap, bool full, const FieldH* fields, const Tile* tiles, \

@@ -31,8 +31,8 @@ hipError_t launch_h_strided_f32(int axis, bool gen, int spec, int ecap, const Fi
 hipError_t launch_h_strided_f64(int axis, bool gen, int spec, int ecap, const FieldH* fields, const Tile* tiles,
                                 int tile0, int ntiles, const PlanDev* plans, const void* pool, size_t lds,
                                 hipStream_t s);
-hipError_t launch_h_kx0_f32(const FieldH* fields, int nfields, int64_t max_pts, hipStream_t s);
-hipError_t launch_h_kx0_f64(const FieldH* fields, int nfields, int64_t max_pts, hipStream_t s);
+hipError_t launch_h_kx0_f32(const FieldH* fields, const int* ids, int nfields, int64_t max_pts, hipStream_t s);
+hipError_t launch_h_kx0_f64(const FieldH* fields, const int* ids, int nfields, int64_t max_pts, hipStream_t s);
 hipError_t launch_h_xpass_f32(int spec, int ecap, bool blend, bool full, const FieldH* fields, const Tile* tiles, int tile0,
                               int ntiles, const PlanDev* plans, const void* pool, MomPartial* partials, size_t lds,
                               hipStream_t s);
@@ -855,8 +855,12 @@ struct scarl_batch {
     // group g + 1 (compute-bound generation) overlaps the y and x passes of group g (HBM-bound).
     struct Group {
         std::vector<TileClass> cz, cy, cx;   // launches of each pass (x: colour-major for the blend)
+        int kx0_off = 0, kx0_cnt = 0;        // its 3D fields in kx0_ids, for k_h_kx0
+        int64_t kx0_max = 0;                 // their largest kx = 0 plane (points)
     };
     std::vector<Group> groups;
+    std::vector<int> kx0_ids;                // the 3D fields of every group, group by group
+    int* d_kx0_ids = nullptr;
     std::vector<hipEvent_t> gev;             // fork event, then one per group (pipelined batches)
     hipEvent_t xev[4] = {nullptr, nullptr, nullptr, nullptr};  // fan-out of a pass: fork, three joins
     std::vector<int2> field_xtiles;          // per-field [begin, end) in tx
@@ -886,7 +890,7 @@ struct scarl_batch {
         for (auto& f : fields)
             if (f.noise_buf) hipFree(f.noise_buf);
         void* ptrs[] = {d_fields, d_tz, d_ty, d_tx, d_field_xtiles, d_partials, d_field_mom, d_slot_count, d_slot_out,
-                        d_ws, d_out, d_kx0};
+                        d_ws, d_out, d_kx0, d_kx0_ids};
         for (void* p : ptrs)
             if (p) hipFree(p);
     }
@@ -1053,6 +1057,7 @@ static void make_tiles(scarl_batch* b) {
     b->ty.clear();
     b->tx.clear();
     b->groups.clear();
+    b->kx0_ids.clear();
     b->field_xtiles.assign(b->fields.size(), make_int2(0, 0));
     std::vector<std::vector<int>> members(1);
     int64_t acc = 0;
@@ -1081,6 +1086,15 @@ static void make_tiles(scarl_batch* b) {
     }
     for (const auto& mem : members) {
         scarl_batch::Group g;
+        g.kx0_off = (int)b->kx0_ids.size();
+        for (int i : mem) {
+            const FieldH& f = b->fields[i].f;
+            if (f.n[2] > 1) {
+                b->kx0_ids.push_back(i);
+                g.kx0_max = std::max<int64_t>(g.kx0_max, (int64_t)f.n[1] * f.n[2]);
+            }
+        }
+        g.kx0_cnt = (int)b->kx0_ids.size() - g.kx0_off;
         std::map<int, std::vector<int>> zp, yp;
         for (int i : mem) {
             const FieldH& f = b->fields[i].f;
@@ -1148,7 +1162,7 @@ static int upload_tables(scarl_batch* b) {
         for (hipEvent_t& e : b->xev) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     for (const PlanDev& pl : b->ps.plans) SCARL_TRY(check_lds((size_t)pl.lds_elems * 2 * rsz));
     void** tabs[] = {(void**)&b->d_tz, (void**)&b->d_ty, (void**)&b->d_tx, (void**)&b->d_field_xtiles,
-                     (void**)&b->d_partials};
+                     (void**)&b->d_partials, (void**)&b->d_kx0_ids};
     for (void** t : tabs)
         if (*t) {
             hipFree(*t);
@@ -1158,6 +1172,7 @@ static int upload_tables(scarl_batch* b) {
     SCARL_TRY(upload_vec(b->ty, (void**)&b->d_ty));
     SCARL_TRY(upload_vec(b->tx, (void**)&b->d_tx));
     SCARL_TRY(upload_vec(b->field_xtiles, (void**)&b->d_field_xtiles));
+    SCARL_TRY(upload_vec(b->kx0_ids, (void**)&b->d_kx0_ids));
     SCARL_TRY(dmalloc((void**)&b->d_partials, std::max<size_t>(1, b->tx.size()) * sizeof(MomPartial)));
     return 0;
 }
@@ -1285,6 +1300,15 @@ static int enqueue_group(scarl_batch* b, const scarl_batch::Group& g, hipStream_
         }
         return 0;
     };
+    // the kx = 0 planes of the group's 3D fields, on the stream of its generating pass just before
+    // it: in a pipelined batch only the first group's then lengthens the step
+    if (g.kx0_cnt > 0 && only_plan < 0) {
+        SCARL_TRY(timed("h_kx0", [&] {
+            const int* ids = b->d_kx0_ids + g.kx0_off;
+            return f64 ? launch_h_kx0_f64(b->d_fields, ids, g.kx0_cnt, g.kx0_max, zs)
+                       : launch_h_kx0_f32(b->d_fields, ids, g.kx0_cnt, g.kx0_max, zs);
+        }));
+    }
     SCARL_TRY(fan_out(g.cz, 0, g.cz.size(), zs, [&](const TC& cl, hipStream_t st) {
         const PlanDev& pl = b->ps.plans[cl.plan];
         return timed("h_zpass", [&] {
@@ -1332,15 +1356,6 @@ static int enqueue_batch(scarl_batch* b) {
     if (pipe) {
         HIP_TRY(hipEventRecord(b->gev[0], s));
         HIP_TRY(hipStreamWaitEvent(g_ctx.gen_stream, b->gev[0], 0));
-    }
-    if (b->kx0_max > 0) {
-        // the kx = 0 planes first, on the stream of the generating passes
-        SCARL_TRY(timed("h_kx0", [&] {
-            const int nf = (int)b->fields.size();
-            hipStream_t zs = pipe ? g_ctx.gen_stream : s;
-            return b->fp64 ? launch_h_kx0_f64(b->d_fields, nf, b->kx0_max, zs)
-                           : launch_h_kx0_f32(b->d_fields, nf, b->kx0_max, zs);
-        }));
     }
     for (size_t g = 0; g < b->groups.size(); ++g)
         SCARL_TRY(enqueue_group(b, b->groups[g], pipe ? g_ctx.gen_stream : s, s, pipe ? b->gev[g + 1] : nullptr));
