@@ -839,7 +839,6 @@ struct FieldHost {
 struct scarl_batch {
     bool fp64 = false;
     bool global = false;
-    bool any_2d = false;  // some field is 2D: its y pass generates the spectrum
     // groups run pipelined (the next group's z pass beside this group's y and x passes) or one
     // after the other with every pass's launches fanned out; see choose_schedule
     bool pipeline = true;
@@ -850,6 +849,7 @@ struct scarl_batch {
     struct TileClass {
         int plan, t0, t1;                    // tiles [t0, t1) of one plan (one kernel launch)
         int color = 0;                       // x pass of a blend: the colour of its subdomains
+        bool gen = false;                    // y pass: the fields are 2D (this pass generates)
     };
     // A group of fields runs its z, y and x passes in order; in a pipelined batch the z pass of
     // group g + 1 (compute-bound generation) overlaps the y and x passes of group g (HBM-bound).
@@ -1095,11 +1095,12 @@ static void make_tiles(scarl_batch* b) {
             }
         }
         g.kx0_cnt = (int)b->kx0_ids.size() - g.kx0_off;
-        std::map<int, std::vector<int>> zp, yp;
+        std::map<int, std::vector<int>> zp;
+        std::map<std::pair<int, int>, std::vector<int>> yp;  // (plan, 2D): 2D fields generate in y
         for (int i : mem) {
             const FieldH& f = b->fields[i].f;
             if (f.n[2] > 1) zp[f.plan_z].push_back(i);
-            yp[f.plan_y].push_back(i);
+            yp[{f.plan_y, f.n[2] == 1 ? 1 : 0}].push_back(i);
         }
         for (auto& kv : zp) {
             const int t0 = (int)b->tz.size(), T = b->ps.plans[kv.first].T;
@@ -1108,10 +1109,12 @@ static void make_tiles(scarl_batch* b) {
             g.cz.push_back({kv.first, t0, (int)b->tz.size()});
         }
         for (auto& kv : yp) {
-            const int t0 = (int)b->ty.size(), T = b->ps.plans[kv.first].T;
+            const int plan = kv.first.first, t0 = (int)b->ty.size(), T = b->ps.plans[plan].T;
             for (int kind : {1, 2, 0})
                 for (int i : kv.second) strided_tiles(i, b->fields[i].f, b->fields[i].f.n[2], T, kind, b->ty);
-            g.cy.push_back({kv.first, t0, (int)b->ty.size()});
+            scarl_batch::TileClass cl{plan, t0, (int)b->ty.size()};
+            cl.gen = kv.first.second != 0;
+            g.cy.push_back(cl);
         }
         for (int c = 0; c < 8; ++c) {
             std::map<int, std::vector<int>> xp;
@@ -1214,8 +1217,6 @@ static int finalize_batch(scarl_batch* b, const std::vector<double>& slot_count)
             return fail(SCARL_E_UNSUPPORTED, "a kz plane of %lld elements exceeds 4 GiB", (long long)f.wz);
         ws_elems += f.wz * f.n[2] + (int64_t)f.nyq * f.n[1] * f.n[2];
     }
-    b->any_2d = false;
-    for (auto& fh : b->fields) b->any_2d = b->any_2d || fh.f.n[2] == 1;
     if (!b->global)
         for (auto& fh : b->fields) out_elems += (int64_t)fh.f.n[0] * fh.f.n[1] * fh.f.n[2];
     else
@@ -1323,7 +1324,7 @@ static int enqueue_group(scarl_batch* b, const scarl_batch::Group& g, hipStream_
     SCARL_TRY(fan_out(g.cy, 0, g.cy.size(), s, [&](const TC& cl, hipStream_t st) {
         const PlanDev& pl = b->ps.plans[cl.plan];
         return timed("h_ypass", [&] {
-            return launch_h_strided(f64, 1, b->any_2d, pl.spec, pl.ecap, b->d_fields, b->d_ty, cl.t0, cl.t1 - cl.t0,
+            return launch_h_strided(f64, 1, cl.gen, pl.spec, pl.ecap, b->d_fields, b->d_ty, cl.t0, cl.t1 - cl.t0,
                                     b->ps.d_plans, b->ps.d_pool, (size_t)pl.lds_elems * csz, st);
         });
     }));

@@ -351,11 +351,9 @@ def test_pipelined_groups_match_single_fields(hip, schedule):
 
 def test_mixed_2d_3d_groups(hip):
     """Three 256^3 fields and two 2D fields in one batch (several groups, the kx = 0 planes of each
-    group's 3D fields generated just before its z pass, 2D fields without one), pipelined and one
-    group after another: both schedules give the same fields bit for bit, through the hipGraph and
-    eagerly, and each field equals the same field generated alone within the FP32 bound (with a 2D
-    field in the batch the y pass of every field runs the kernel that can generate, which stages
-    its pencils through LDS, so the rounding may differ from the field alone in the last place)."""
+    group's 3D fields generated just before its z pass, 2D fields without one; the 2D fields' y
+    launches generate, the 3D fields' do not), pipelined and one group after another: every field
+    equals, bit for bit, the same field generated alone, through the hipGraph and eagerly."""
     from scarl_amd import _capi as A, generator
     shapes = [(256, 256, 256), (128, 96, 1), (256, 256, 256), (96, 160, 1), (256, 256, 256)]
     specs = [generator.FieldSpec(n=n, model=1 + i % 3, corrL=(12.0, 10.0, 8.0), nu=0.4, seed=8, stream=i, margi=2,
@@ -365,7 +363,6 @@ def test_mixed_2d_3d_groups(hip):
         with generator.Batch.ensemble([sp], 32) as s:
             s.run()
             alone.append(s.output(0))
-    first = None
     for schedule in (1, 2):
         generator.set_tuning(A.TUNE_SCHEDULE, schedule)
         try:
@@ -377,12 +374,8 @@ def test_mixed_2d_3d_groups(hip):
             for use_graph in (True, False):
                 b.run(use_graph=use_graph)
                 b.sync()
-                outs = [b.output(i) for i in range(len(specs))]
-                if first is None:
-                    first = outs
                 for i in range(len(specs)):
-                    np.testing.assert_array_equal(outs[i], first[i])
-                    assert rel_l2(outs[i], alone[i]) <= 1e-6, (schedule, i)
+                    np.testing.assert_array_equal(b.output(i), alone[i])
 
 
 def _blend_reference(gdim, starts, exts, spec, dx=(1, 1, 1)):
