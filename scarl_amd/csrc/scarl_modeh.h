@@ -419,6 +419,15 @@ __global__ __launch_bounds__(kBlock) void k_h_xpass(const FieldH* __restrict__ f
         // 2-element store when the row is aligned for it.
         // odd n1: element k of the pair's two rows (the second may be absent)
         const int ne = full ? (c.o2 ? 2 : 1) : 2;
+        // even n1, aligned row: what the earlier owners stored at the element's two outputs comes
+        // in one 8-byte load (the lanes' pairs are contiguous) wherever either output needs it
+        const bool pair_ok = !full && (reinterpret_cast<uintptr_t>(o) % (2 * sizeof(T))) == 0;
+        cplx<T> prev2{T(0), T(0)};
+        if (pair_ok) {
+            const int m0 = c.myz | (in_overlap(2 * k, n1, f.nov_lo[0], f.nov_hi[0]) ? 1 : 0);
+            const int m1 = c.myz | (in_overlap(2 * k + 1, n1, f.nov_lo[0], f.nov_hi[0]) ? 1 : 0);
+            if (f.color & (m0 | m1)) prev2 = gload(reinterpret_cast<const cplx<T>*>(o + 2 * k));
+        }
         T val[2];
 #pragma unroll
         for (int e = 0; e < 2; ++e) {
@@ -428,7 +437,8 @@ __global__ __launch_bounds__(kBlock) void k_h_xpass(const FieldH* __restrict__ f
             T* oe = r2 ? c.o2 : o;
             const int msk = (r2 ? c.myz2 : c.myz) | (in_overlap(i, n1, f.nov_lo[0], f.nov_hi[0]) ? 1 : 0);
             const T v0 = (e ? z.y : z.x) * ((r2 ? c.swyz2 : c.swyz) * wxl[i]);
-            const T v = (f.color & msk) ? v0 + gload1(oe + i) : v0;  // not the first owner: add
+            const T prev = pair_ok ? (e ? prev2.y : prev2.x) : (f.color & msk) ? gload1(oe + i) : T(0);
+            const T v = (f.color & msk) ? v0 + prev : v0;  // not the first owner: add
             const bool last = (f.color & msk) == msk;
             const T r = apply_margi<T>(v, f.margi, mu, sd);
             val[e] = last ? r : v;
